@@ -1,0 +1,25 @@
+"""cubie_amd: MI355X-native batch ODE integration behind CuBIE's Python API.
+
+Public entry points mirror the reference package (ccam80/cubie): ``solve_ivp``,
+``Solver``, ``create_ODE_system`` / ``SymbolicODE``, ``ArrayInterpolator``,
+``summary_metrics`` / ``register_metric`` and ``CUBIE_RESULT_CODES``. The
+integration itself runs in HIP kernels compiled for gfx950 (see
+``cubie_amd.csrc``); there is no CPU fallback.
+"""
+
+from .array_interpolator import ArrayInterpolator
+from .batchsolving.solver import Solver, solve_ivp
+from .batchsolving.solveresult import DeviceSolveResult, SolveResult
+from .integrators import tableaus
+from .integrators.algorithms import algorithm_is_adaptive, resolve_algorithm
+from .odesystems.symbolic import SymbolicODE, create_ODE_system
+from .outputhandling.summary_metrics import SummaryMetric, register_metric, summary_metrics
+from .result_codes import CUBIE_RESULT_CODES, decode_status_codes
+
+__all__ = [
+    "ArrayInterpolator", "Solver", "solve_ivp", "SolveResult", "DeviceSolveResult",
+    "SymbolicODE", "create_ODE_system", "summary_metrics", "register_metric",
+    "SummaryMetric", "CUBIE_RESULT_CODES", "decode_status_codes", "tableaus",
+    "algorithm_is_adaptive", "resolve_algorithm",
+]
+__version__ = "0.1.0"

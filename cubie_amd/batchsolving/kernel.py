@@ -1,0 +1,349 @@
+"""Batch kernel: settings -> compiled HIP module -> launch on device buffers.
+
+Host-side counterpart of the reference's BatchSolverKernel
+(batchsolving/BatchSolverKernel.py:208-1010) and SingleIntegratorRunCore
+(integrators/SingleIntegratorRunCore.py:109-414): it resolves the algorithm
+and controller (with the errorless-algorithm fallback to a fixed controller,
+:435-513), derives the inner Newton / Krylov tolerances from the controller
+(:353-414), validates the timing (:457-523), sizes the outputs in the
+(time, variable, run) layout, casts the run times to the working precision
+and computes the save / summary stop times (:706-722) before launching.
+
+Device buffers are torch tensors (allocation and streams only); the launch
+goes through the C ABI (cubie_amd.runtime.native). Two kernels come from one
+module: ``cubie_integrate`` (one lane per system) and
+``cubie_integrate_persistent`` (GPU-filling grid with an atomic work queue).
+"""
+
+from __future__ import annotations
+
+import math
+import warnings
+from dataclasses import dataclass
+from typing import Any, Dict, Optional
+
+import numpy as np
+
+from ..integrators.algorithms import resolve_algorithm
+from ..integrators.controllers import make_controller
+from ..integrators.kernel_source import ImplicitSettings, KernelSpec
+from ..integrators.timing import LoopTiming
+from ..outputhandling.output_config import OutputConfig
+from ..runtime import kcache, native
+
+KERNEL_MODES = ("plain", "persistent")
+PRECISION_MODES = ("plain", "compensated")
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise native.CubieNativeError("No HIP device is available for the batch kernel")
+    return torch
+
+
+@dataclass
+class DeviceOutputs:
+    state: Any
+    observables: Any
+    state_summaries: Any
+    observable_summaries: Any
+    iteration_counters: Any
+    status_codes: Any
+
+
+class BatchSolverKernel:
+    def __init__(self, system, algorithm="euler", step_control_settings=None,
+                 algorithm_settings=None, output_settings=None, loop_settings=None,
+                 kernel_mode: str = "plain", precision_mode: str = "plain",
+                 fp_contract: Optional[bool] = None, block_size: int = 256,
+                 blocks_per_cu: int = 4):
+        self.system = system
+        self.precision = system.precision
+        if kernel_mode not in KERNEL_MODES:
+            raise ValueError(f"kernel_mode must be one of {KERNEL_MODES}")
+        if precision_mode not in PRECISION_MODES:
+            raise ValueError(f"precision_mode must be one of {PRECISION_MODES}")
+        if precision_mode == "compensated" and self.precision is not np.float32:
+            raise ValueError("precision_mode='compensated' needs a float32 system")
+        self.kernel_mode = kernel_mode
+        self.precision_mode = precision_mode
+        self.block_size = int(block_size)
+        self.blocks_per_cu = int(blocks_per_cu)
+        self.fp_contract = bool(fp_contract) if fp_contract is not None else False
+        self.driver_interpolator = None
+        self._algorithm_arg = algorithm
+        self._step_settings = dict(step_control_settings or {})
+        self._algo_settings = dict(algorithm_settings or {})
+        self._output_settings = dict(output_settings or {})
+        self._loop_settings = dict(loop_settings or {})
+        self._configure()
+
+    # -- configuration ---------------------------------------------------------
+    def _configure(self) -> None:
+        s = self.system
+        n = s.num_states
+        self.algorithm = resolve_algorithm(self._algorithm_arg)
+        ctrl_settings = self.algorithm.controller_defaults()
+        ctrl_settings.update({k: v for k, v in self._step_settings.items() if v is not None})
+        if "dt" in self._loop_settings and "dt" not in ctrl_settings:
+            ctrl_settings["dt"] = self._loop_settings["dt"]
+        self.controller = make_controller(ctrl_settings, self.precision, n, self.algorithm.order)
+        if not self.algorithm.is_adaptive and self.controller.is_adaptive:
+            warnings.warn(
+                f"Adaptive step controller '{self.controller.kind}' cannot be used with "
+                f"fixed-step algorithm '{self.algorithm.name}'. The algorithm does not "
+                "provide an error estimate required for adaptive stepping. Replacing "
+                f"with fixed-step controller (dt={self.controller.dt}).", UserWarning, stacklevel=3)
+            self.controller = make_controller(
+                {"step_controller": "fixed", "dt": self.controller.dt,
+                 "atol": self.controller.atol, "rtol": self.controller.rtol},
+                self.precision, n, self.algorithm.order)
+        self.implicit = self._implicit_settings()
+        os_ = dict(self._output_settings)
+        self.output_config = OutputConfig(
+            max_states=n, max_observables=s.num_observables,
+            output_types=os_.get("output_types", ["state"]),
+            saved_state_indices=os_.get("saved_state_indices"),
+            saved_observable_indices=os_.get("saved_observable_indices"),
+            summarised_state_indices=os_.get("summarised_state_indices"),
+            summarised_observable_indices=os_.get("summarised_observable_indices"))
+        ls = self._loop_settings
+        self.timing = LoopTiming.resolve(
+            self.precision, ls.get("save_every"), ls.get("summarise_every"),
+            ls.get("sample_summaries_every") or self._output_settings.get("sample_summaries_every"),
+            self.output_config.time_domain_outputs, self.output_config.save_summaries)
+        self._spec_cache: Dict[Any, KernelSpec] = {}
+
+    def _implicit_settings(self) -> ImplicitSettings:
+        im = ImplicitSettings()
+        a = self._algo_settings
+        ctrl = self.controller
+        if self.algorithm.is_implicit:
+            # SingleIntegratorRunCore._apply_inner_tolerance_defaults
+            im.krylov_atol = float(ctrl.atol.min())
+            im.krylov_rtol = float(ctrl.rtol.min())
+            im.newton_atol = float(ctrl.atol.min()) / 10.0
+            im.newton_rtol = float(ctrl.rtol.min()) / 10.0
+            floor = float(ctrl.rtol.min())
+            if ctrl.is_adaptive and floor > 0.0:
+                if self.algorithm.is_linear:
+                    floor *= 0.01
+                im.krylov_residual_reduction = floor
+            else:
+                im.krylov_residual_reduction = float(np.finfo(self.precision).eps)
+        for key in ("newton_atol", "newton_rtol", "krylov_atol", "krylov_rtol",
+                    "krylov_residual_reduction"):
+            if a.get(key) is not None:
+                setattr(im, key, float(np.min(a[key])))
+        if a.get("krylov_residual_floor") is not None:
+            im.krylov_residual_floor = float(a["krylov_residual_floor"])
+        for key in ("newton_max_iters", "krylov_max_iters", "preconditioner_order"):
+            if a.get(key) is not None:
+                setattr(im, key, int(a[key]))
+        if a.get("linear_correction_type") is not None:
+            im.linear_correction_type = str(a["linear_correction_type"])
+        return im
+
+    def update(self, updates: Dict[str, Any]) -> set:
+        """Apply setting updates; returns the recognised keys."""
+        from ..integrators.controllers import ALL_STEP_CONTROLLER_PARAMETERS
+        from ..integrators.algorithms import ALL_ALGORITHM_STEP_PARAMETERS
+        from ..integrators.loop_settings import ALL_LOOP_SETTINGS
+        from ..outputhandling.output_config import ALL_OUTPUT_FUNCTION_PARAMETERS
+        rec = set()
+        for k, v in updates.items():
+            if k == "algorithm":
+                self._algorithm_arg = v
+                rec.add(k)
+            elif k in ALL_STEP_CONTROLLER_PARAMETERS:
+                self._step_settings[k] = v
+                rec.add(k)
+            elif k in ALL_ALGORITHM_STEP_PARAMETERS:
+                self._algo_settings[k] = v
+                rec.add(k)
+            elif k in ALL_LOOP_SETTINGS:
+                self._loop_settings[k] = v
+                if k == "dt":
+                    self._step_settings["dt"] = v
+                rec.add(k)
+            elif k in ALL_OUTPUT_FUNCTION_PARAMETERS:
+                self._output_settings[k] = v
+                rec.add(k)
+            elif k in ("kernel_mode", "precision_mode", "block_size", "blocks_per_cu", "fp_contract"):
+                setattr(self, k, v)
+                rec.add(k)
+        if rec:
+            self._configure()
+        return rec
+
+    # -- timing validation (BatchSolverKernel._validate_timing_parameters) ----
+    def _validate_timing(self, timing: LoopTiming, duration: float) -> None:
+        end_time = float(self.precision(duration)) + float(self.precision(self.controller.dt_min))
+        oc = self.output_config
+        if oc.time_domain_outputs and timing.save_every is not None:
+            if timing.save_every_p > end_time and not timing.save_last:
+                raise ValueError(f"save_every ({timing.save_every}) > duration ({duration}) so "
+                                 "this loop will produce no outputs")
+        if oc.save_summaries:
+            if timing.sample_summaries_every is None:
+                raise ValueError("Summary outputs are enabled but sample_summaries_every is None")
+            if timing.summarise_every is None:
+                raise ValueError("Summary outputs are enabled but summarise_every is None")
+            if timing.sample_every_p >= timing.summarise_every_p:
+                raise ValueError(
+                    f"sample_summaries_every ({timing.sample_summaries_every}) >= summarise_every "
+                    f"({timing.summarise_every}); The saved summary will be based on 0 samples")
+            if timing.summarise_every_p > end_time:
+                raise ValueError(f"summarise_every ({timing.summarise_every}) > duration "
+                                 f"({duration}), so this loop will produce no summary outputs")
+
+    # -- spec / module --------------------------------------------------------
+    def spec_for(self, duration: float, per_run_rtol: bool = False) -> KernelSpec:
+        timing = self.timing.with_duration(duration)
+        key = (timing.summarise_every, timing.sample_summaries_every, per_run_rtol,
+               self.precision_mode, self.fp_contract, self.block_size)
+        if key not in self._spec_cache:
+            self._spec_cache[key] = KernelSpec(
+                system=self.system, precision=self.precision, algorithm=self.algorithm,
+                controller=self.controller, timing=timing, outputs=self.output_config,
+                drivers=self.driver_interpolator, implicit=self.implicit,
+                compensated=self.precision_mode == "compensated", per_run_rtol=per_run_rtol,
+                fp_contract=self.fp_contract, block_size=self.block_size)
+        return self._spec_cache[key]
+
+    def module(self, spec: KernelSpec, kernel_mode: Optional[str] = None):
+        mode = kernel_mode or self.kernel_mode
+        name = "cubie_integrate" if mode == "plain" else "cubie_integrate_persistent"
+        return kcache.get_module(spec.source(), spec.compile_options(), name)
+
+    def compile_only(self, duration: float = 1.0, per_run_rtol: bool = False) -> bytes:
+        """Compile (or fetch from the cache) without touching a GPU."""
+        spec = self.spec_for(duration, per_run_rtol)
+        _, code = kcache.get_code(spec.source(), spec.compile_options())
+        return code
+
+    # -- run --------------------------------------------------------------------
+    def output_shapes(self, timing: LoopTiming, duration: float, n_runs: int) -> Dict[str, tuple]:
+        oc = self.output_config
+        rows = timing.output_length(duration) if oc.time_domain_outputs else 0
+        srows = timing.summaries_length(duration) if oc.save_summaries else 0
+        oh = oc.summaries_output_height_per_var
+        return {
+            "state": (rows, oc.state_output_width, n_runs) if (oc.save_state or oc.save_time) else (0, 0, n_runs),
+            "observables": (rows, oc.n_saved_observables, n_runs) if oc.save_observables else (0, 0, n_runs),
+            "state_summaries": (srows, oc.n_summarised_states * oh, n_runs) if oc.summarise_state else (0, 0, n_runs),
+            "observable_summaries": (srows, oc.n_summarised_observables * oh, n_runs) if oc.summarise_observables else (0, 0, n_runs),
+            "iteration_counters": (rows, 4, n_runs) if oc.save_counters else (0, 0, n_runs),
+        }
+
+    def run(self, inits, params, duration: float, warmup: float = 0.0, t0: float = 0.0,
+            blocksize: Optional[int] = None, rtol_per_run=None, duration_per_run=None,
+            kernel_mode: Optional[str] = None, timed: bool = False):
+        """Integrate the batch; inputs are (n_vars, n_runs) host or torch arrays.
+
+        Returns (DeviceOutputs, kernel_ms or None, launch_info).
+        """
+        torch = _torch()
+        prec = self.precision
+        tdt = torch.float32 if prec is np.float32 else torch.float64
+        dev = torch.device("cuda")
+
+        def to_dev(x, dtype):
+            if isinstance(x, torch.Tensor):
+                return x.to(device=dev, dtype=dtype).contiguous()
+            np_dt = np.float32 if dtype == torch.float32 else np.float64
+            return torch.from_numpy(np.ascontiguousarray(x, dtype=np_dt)).to(dev)
+
+        d_inits = to_dev(inits, tdt)
+        d_params = to_dev(params, tdt)
+        n_runs = int(d_inits.shape[1]) if d_inits.ndim == 2 else 1
+        if self.system.num_parameters and d_params.shape[1] != n_runs:
+            raise ValueError("inits and params must have the same number of runs")
+        duration = float(np.float64(duration))
+        timing = self.timing.with_duration(duration)
+        self._validate_timing(timing, duration)
+        per_run_rtol = rtol_per_run is not None
+        spec = self.spec_for(duration, per_run_rtol)
+        mode = kernel_mode or self.kernel_mode
+        mod = self.module(spec, mode)
+
+        # run times cast to the working precision before the launch
+        dur_p = float(prec(duration))
+        warm_p = float(prec(warmup))
+        t0_p = float(prec(t0))
+        if duration_per_run is not None:
+            dpr = np.asarray(duration_per_run, dtype=np.float64).reshape(-1)
+            if dpr.size != n_runs:
+                raise ValueError("duration_per_run must have one entry per run")
+            dur_for_rows = float(dpr.max())
+        else:
+            dur_for_rows = dur_p
+        save_stop = float(prec(timing.save_stop_time(dur_for_rows, warm_p, t0_p)))
+        summary_stop = float(prec(timing.summary_stop_time(dur_for_rows, warm_p, t0_p)))
+        if duration_per_run is not None and (timing.save_regularly or timing.summarise_regularly):
+            raise ValueError("duration_per_run requires save_last / summary-free outputs")
+
+        shapes = self.output_shapes(timing, dur_for_rows, n_runs)
+        outs = {}
+        for k, shp in shapes.items():
+            dt_ = torch.int32 if k == "iteration_counters" else tdt
+            outs[k] = torch.zeros(shp, dtype=dt_, device=dev) if shp[0] * shp[1] > 0 else torch.zeros(1, dtype=dt_, device=dev)
+        status = torch.zeros(n_runs, dtype=torch.int32, device=dev)
+        coeffs = None
+        if self.system.num_drivers > 0:
+            if self.driver_interpolator is None:
+                raise ValueError("The system has drivers but no driver arrays were supplied")
+            coeffs = torch.from_numpy(np.ascontiguousarray(self.driver_interpolator.coefficients)).to(dev)
+        rtol_d = to_dev(np.asarray(rtol_per_run), tdt) if per_run_rtol else None
+        dur_d = (torch.from_numpy(np.asarray(duration_per_run, dtype=np.float64)).to(dev)
+                 if duration_per_run is not None else None)
+        counter = torch.zeros(1, dtype=torch.int32, device=dev)
+
+        args = native.SolveArgs()
+        args.inits = d_inits.data_ptr()
+        args.params = d_params.data_ptr() if d_params.numel() else 0
+        args.driver_coeffs = coeffs.data_ptr() if coeffs is not None else 0
+        args.state_out = outs["state"].data_ptr()
+        args.observables_out = outs["observables"].data_ptr()
+        args.state_summ_out = outs["state_summaries"].data_ptr()
+        args.obs_summ_out = outs["observable_summaries"].data_ptr()
+        args.counters_out = outs["iteration_counters"].data_ptr()
+        args.status_out = status.data_ptr()
+        args.rtol_per_run = rtol_d.data_ptr() if rtol_d is not None else 0
+        args.duration_per_run = dur_d.data_ptr() if dur_d is not None else 0
+        args.save_times = 0
+        args.work_counter = counter.data_ptr()
+        args.checkpoints = 0
+        args.checkpoint_stride = 0
+        args.duration = dur_p
+        args.warmup = warm_p
+        args.t0 = t0_p
+        args.save_stop = save_stop
+        args.summary_stop = summary_stop
+        args.n_runs = n_runs
+        args.out_rows = shapes["state"][0] if shapes["state"][1] else (
+            shapes["observables"][0] if shapes["observables"][1] else shapes["iteration_counters"][0])
+        args.summary_rows = max(shapes["state_summaries"][0], shapes["observable_summaries"][0])
+        args.chunk_runs = 1
+        args.reserved = 0
+
+        block = int(blocksize or self.block_size)
+        if block != spec.block_size:
+            block = spec.block_size
+        if mode == "plain":
+            grid = max(1, math.ceil(n_runs / block))
+        else:
+            cus = torch.cuda.get_device_properties(dev).multi_processor_count
+            grid = max(1, min(math.ceil(n_runs / block), cus * self.blocks_per_cu))
+        stream = torch.cuda.current_stream().cuda_stream
+        ms = None
+        if timed:
+            ms = mod.launch_timed(args, grid, block, stream)
+        else:
+            mod.launch(args, grid, block, stream)
+        dout = DeviceOutputs(outs["state"], outs["observables"], outs["state_summaries"],
+                             outs["observable_summaries"], outs["iteration_counters"], status)
+        info = {"grid": grid, "block": block, "mode": mode, "shapes": shapes, "spec": spec,
+                "module": mod, "args": args, "keepalive": (d_inits, d_params, coeffs, rtol_d, dur_d, counter)}
+        return dout, ms, info

@@ -1,0 +1,287 @@
+"""User API: ``solve_ivp`` and ``Solver``.
+
+Same signatures, keyword routing and error behaviour as the reference
+(batchsolving/solver.py:215-854): keyword arguments are routed to the output,
+controller, algorithm and loop setting groups by the ``ALL_*`` name sets;
+unknown keywords raise ``KeyError``; the legacy timing names ``dt_save``,
+``dt_summarise`` and ``dt_update_summaries`` raise ``KeyError`` with a rename
+hint; ``save_variables`` / ``summarise_variables`` labels are converted to
+index arrays (``None`` = all, ``[]`` = none); dict inputs build a run grid,
+(n_vars, n_runs) arrays are used as given; drivers are a dict of sample
+arrays plus interpolation settings. ``solve_ivp`` builds the system from
+equations when given strings or a callable.
+
+Engine-specific keywords (not in the reference): ``kernel_mode``
+("plain" | "persistent"), ``precision_mode`` ("plain" | "compensated"),
+``fp_contract``, ``block_size``, ``blocks_per_cu`` and, at solve time,
+``rtol_per_run`` / ``duration_per_run`` arrays.
+"""
+
+from __future__ import annotations
+
+from typing import Any, Dict, Iterable, List, Optional, Set, Union
+
+import numpy as np
+
+from ..array_interpolator import ArrayInterpolator
+from ..integrators.algorithms import ALL_ALGORITHM_STEP_PARAMETERS
+from ..integrators.controllers import ALL_STEP_CONTROLLER_PARAMETERS
+from ..integrators.loop_settings import ALL_LOOP_SETTINGS
+from ..odesystems.symbolic import SymbolicODE, create_ODE_system
+from ..outputhandling.output_config import ALL_OUTPUT_FUNCTION_PARAMETERS
+from .input_handler import BatchInputHandler
+from .kernel import BatchSolverKernel
+from .solveresult import DeviceSolveResult, SolveResult
+
+RENAMED_TIMING_KWARGS = {
+    "dt_save": "save_every",
+    "dt_summarise": "summarise_every",
+    "dt_update_summaries": "sample_summaries_every",
+}
+ALL_MEMORY_MANAGER_PARAMETERS = {"memory_manager", "stream_group", "mem_proportion",
+                                 "host_spill_threshold", "spill_directory"}
+ALL_CACHE_PARAMETERS = {"cache", "cache_dir", "cache_mode", "max_cache_entries"}
+ENGINE_PARAMETERS = {"kernel_mode", "precision_mode", "fp_contract", "block_size", "blocks_per_cu"}
+
+
+def _check_renamed(keys: Iterable[str]) -> None:
+    renamed = [k for k in keys if k in RENAMED_TIMING_KWARGS]
+    if renamed:
+        hints = ", ".join(f"'{k}' is now '{RENAMED_TIMING_KWARGS[k]}'" for k in renamed)
+        raise KeyError(f"Renamed keyword argument(s): {hints}.")
+
+
+def _split(kwargs: Dict[str, Any], keys: Set[str]) -> Dict[str, Any]:
+    out = {}
+    for k in list(kwargs):
+        if k in keys:
+            out[k] = kwargs.pop(k)
+    return out
+
+
+class Solver:
+    """Batch solver for one system (reference solver.py:350-854)."""
+
+    def __init__(self, system: SymbolicODE, algorithm="euler", lineinfo=None,
+                 step_control_settings=None, algorithm_settings=None, output_settings=None,
+                 memory_settings=None, loop_settings=None, time_logging_level=None,
+                 cache=True, auto_memory=True, **kwargs):
+        _check_renamed(kwargs)
+        self.system = system
+        kwargs = dict(kwargs)
+        kwargs.pop("precision", None)
+        out_s = dict(output_settings or {})
+        out_s.update(_split(kwargs, ALL_OUTPUT_FUNCTION_PARAMETERS))
+        labels = _split(kwargs, {"save_variables", "summarise_variables"})
+        out_s.update(labels)
+        self._convert_output_labels(out_s)
+        _split(kwargs, ALL_MEMORY_MANAGER_PARAMETERS | ALL_CACHE_PARAMETERS)
+        step_s = dict(step_control_settings or {})
+        step_s.update(_split(kwargs, ALL_STEP_CONTROLLER_PARAMETERS))
+        algo_s = dict(algorithm_settings or {})
+        algo_s.update(_split(kwargs, ALL_ALGORITHM_STEP_PARAMETERS - {"algorithm"}))
+        loop_s = dict(loop_settings or {})
+        loop_s.update(_split(kwargs, ALL_LOOP_SETTINGS))
+        if "dt" in loop_s:
+            step_s.setdefault("dt", loop_s["dt"])
+        eng = _split(kwargs, ENGINE_PARAMETERS)
+        if kwargs:
+            raise KeyError(f"Unrecognized keyword arguments: {set(kwargs)}")
+        self.kernel = BatchSolverKernel(system, algorithm=algorithm,
+                                        step_control_settings=step_s, algorithm_settings=algo_s,
+                                        output_settings=out_s, loop_settings=loop_s, **eng)
+        self.input_handler = BatchInputHandler(system)
+        self.driver_interpolator: Optional[ArrayInterpolator] = None
+        if system.driver_defaults:
+            self._configure_drivers(system.driver_defaults)
+        self._last = None
+
+    # -- labels ------------------------------------------------------------------
+    def _convert_output_labels(self, s: Dict[str, Any]) -> None:
+        sysm = self.system
+        for key, st_key, ob_key in (("save_variables", "saved_state_indices", "saved_observable_indices"),
+                                    ("summarise_variables", "summarised_state_indices",
+                                     "summarised_observable_indices")):
+            if key not in s:
+                continue
+            labels = s.pop(key)
+            if labels is None:
+                continue
+            st_idx, ob_idx = [], []
+            for lab in labels:
+                if lab in sysm.state_names:
+                    st_idx.append(sysm.state_names.index(lab))
+                elif lab in sysm.observable_names:
+                    ob_idx.append(sysm.observable_names.index(lab))
+                else:
+                    raise ValueError(f"Variable label '{lab}' is not a state or observable")
+            prev_s = s.get(st_key)
+            prev_o = s.get(ob_key)
+            if prev_s is not None:
+                st_idx = sorted(set(st_idx) | set(int(i) for i in prev_s))
+            if prev_o is not None:
+                ob_idx = sorted(set(ob_idx) | set(int(i) for i in prev_o))
+            s[st_key] = st_idx
+            s[ob_key] = ob_idx
+
+    def _configure_drivers(self, drivers: Dict[str, Any]) -> None:
+        if self.driver_interpolator is None:
+            self.driver_interpolator = ArrayInterpolator(self.system.precision, drivers,
+                                                         driver_names=self.system.driver_names)
+        else:
+            self.driver_interpolator.update_from_dict(drivers)
+        self.kernel.driver_interpolator = self.driver_interpolator
+        self.kernel._spec_cache.clear()
+
+    # -- API ---------------------------------------------------------------------
+    def build_grid(self, initial_values=None, parameters=None, grid_type="verbatim"):
+        return self.input_handler(states=initial_values, params=parameters, kind=grid_type)
+
+    def update(self, updates_dict=None, silent=False, **kwargs) -> Set[str]:
+        upd = dict(updates_dict or {})
+        upd.update(kwargs)
+        if not upd:
+            return set()
+        _check_renamed(upd)
+        if "save_variables" in upd or "summarise_variables" in upd:
+            self._convert_output_labels(upd)
+        rec = self.kernel.update(upd)
+        unrec = set(upd) - rec
+        if unrec and not silent:
+            raise KeyError(f"Unrecognized parameters: {unrec}")
+        return rec
+
+    def solve(self, initial_values, parameters, drivers=None, duration=1.0, settling_time=0.0,
+              t0=0.0, blocksize=256, grid_type="verbatim", nan_error_trajectories=True,
+              on_device=False, rtol_per_run=None, duration_per_run=None, **kwargs):
+        if kwargs:
+            self.update(kwargs)
+        inits, params = self.input_handler(states=initial_values, params=parameters, kind=grid_type)
+        if drivers is not None:
+            self._configure_drivers(drivers)
+        dout, ms, info = self.kernel.run(inits, params, duration=duration, warmup=settling_time,
+                                         t0=t0, blocksize=blocksize, rtol_per_run=rtol_per_run,
+                                         duration_per_run=duration_per_run, timed=True)
+        self._last = info
+        oc = self.kernel.output_config
+        active = {"state": oc.save_state, "observables": oc.save_observables,
+                  "state_summaries": oc.summarise_state,
+                  "observable_summaries": oc.summarise_observables,
+                  "iteration_counters": oc.save_counters}
+        if on_device:
+            import torch
+            return DeviceSolveResult(dout, torch.cuda.current_stream(), oc.save_time, active)
+        host = {k: getattr(dout, k).cpu().numpy() for k in
+                ("state", "observables", "state_summaries", "observable_summaries",
+                 "iteration_counters", "status_codes")}
+        sh = info["shapes"]
+        for k in ("state", "observables", "state_summaries", "observable_summaries",
+                  "iteration_counters"):
+            host[k] = host[k].reshape(sh[k]) if np.prod(sh[k]) > 0 else np.zeros(sh[k], dtype=host[k].dtype)
+        return SolveResult(host["state"], host["observables"], host["state_summaries"],
+                           host["observable_summaries"], host["status_codes"],
+                           host["iteration_counters"], save_time=oc.save_time, active=active,
+                           time_domain_legend=self.time_domain_legend,
+                           summaries_legend=self.summaries_legend,
+                           solve_settings={"duration": duration, "t0": t0,
+                                           "settling_time": settling_time},
+                           nan_error_trajectories=nan_error_trajectories, kernel_ms=ms)
+
+    @property
+    def time_domain_legend(self) -> Dict[int, str]:
+        oc = self.kernel.output_config
+        s = self.system
+        leg = {}
+        i = 0
+        if oc.save_state:
+            for k in oc.saved_state_indices:
+                leg[i] = s.state_names[int(k)]
+                i += 1
+        if oc.save_observables:
+            for k in oc.saved_observable_indices:
+                leg[i] = s.observable_names[int(k)]
+                i += 1
+        return leg
+
+    @property
+    def summaries_legend(self) -> Dict[int, str]:
+        oc = self.kernel.output_config
+        s = self.system
+        per = oc.summary_legend_per_variable
+        leg = {}
+        i = 0
+        if oc.summarise_state:
+            for k in oc.summarised_state_indices:
+                for m in per:
+                    leg[i] = f"{s.state_names[int(k)]} {m}"
+                    i += 1
+        if oc.summarise_observables:
+            for k in oc.summarised_observable_indices:
+                for m in per:
+                    leg[i] = f"{s.observable_names[int(k)]} {m}"
+                    i += 1
+        return leg
+
+    @property
+    def output_types(self):
+        return self.kernel.output_config.output_types
+
+    @property
+    def precision(self):
+        return self.system.precision
+
+    def close(self, shutdown_timeout=None) -> None:
+        pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _system_from_equations(dxdt, y0, parameters, drivers, precision=None) -> SymbolicODE:
+    if parameters is not None and not isinstance(parameters, dict):
+        raise TypeError("When equations are supplied directly to solve_ivp, parameters must "
+                        "be a dict mapping names to values so the system's parameters can "
+                        "be declared.")
+    states = None
+    if isinstance(y0, dict):
+        states = {k: float(np.asarray(v).flat[0]) for k, v in y0.items()}
+    pdef = None
+    if isinstance(parameters, dict):
+        pdef = {k: float(np.asarray(v).flat[0]) for k, v in parameters.items()}
+    kw = {}
+    if precision is not None:
+        kw["precision"] = precision
+    drv = None
+    if drivers is not None:
+        drv = [k for k in drivers if k not in ("time", "dt", "wrap", "order",
+                                               "boundary_condition", "t0",
+                                               "driver_sample_period")]
+    return create_ODE_system(dxdt=dxdt, states=states, parameters=pdef, drivers=drv, **kw)
+
+
+def solve_ivp(system, y0, parameters=None, drivers=None, method="euler", duration=1.0,
+              settling_time=0.0, t0=0.0, save_variables=None, summarise_variables=None,
+              grid_type="combinatorial", time_logging_level=None, nan_error_trajectories=True,
+              **kwargs) -> SolveResult:
+    """Solve a batch initial value problem (reference solver.py:215-347)."""
+    if not isinstance(system, SymbolicODE):
+        system = _system_from_equations(system, y0, parameters, drivers,
+                                        precision=kwargs.pop("precision", None))
+    loop_settings = kwargs.pop("loop_settings", None)
+    if save_variables is not None:
+        kwargs.setdefault("save_variables", save_variables)
+    if summarise_variables is not None:
+        kwargs.setdefault("summarise_variables", summarise_variables)
+    solve_opts = {k: kwargs.pop(k) for k in ("blocksize", "rtol_per_run", "duration_per_run")
+                  if k in kwargs}
+    solver = Solver(system, algorithm=method, loop_settings=loop_settings,
+                    time_logging_level=time_logging_level, **kwargs)
+    try:
+        return solver.solve(y0, parameters, drivers=drivers, duration=duration,
+                            settling_time=settling_time, t0=t0, grid_type=grid_type,
+                            nan_error_trajectories=nan_error_trajectories, **solve_opts)
+    finally:
+        solver.close()

@@ -1,0 +1,231 @@
+// Host side of the C ABI declared in include/cubie_hip.h.
+//
+// hiprtc turns the generated translation unit into a gfx950 code object;
+// hipModule* loads it and launches the integration kernel on the caller's
+// stream. Nothing here allocates device memory: buffers are owned by the
+// caller (the Python package allocates them through torch).
+
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cubie_hip.h"
+
+struct cubie_module {
+    hipModule_t module;
+    hipFunction_t function;
+    std::string kernel_name;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t err, const char* what) {
+    std::string msg = std::string(what) + ": " + hipGetErrorName(err) + " (" +
+                      hipGetErrorString(err) + ")";
+    return fail(CUBIE_E_HIP, msg);
+}
+
+std::vector<std::string> split_options(const char* options) {
+    std::vector<std::string> out;
+    if (!options) return out;
+    std::string cur;
+    for (const char* p = options; *p; ++p) {
+        if (*p == '\n') {
+            if (!cur.empty()) out.push_back(cur);
+            cur.clear();
+        } else {
+            cur.push_back(*p);
+        }
+    }
+    if (!cur.empty()) out.push_back(cur);
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cubie_hip_abi_version(void) { return CUBIE_ABI_VERSION; }
+
+int cubie_hip_last_error(char* buf, size_t len) {
+    if (buf && len > 0) {
+        std::strncpy(buf, g_last_error.c_str(), len - 1);
+        buf[len - 1] = '\0';
+    }
+    return static_cast<int>(g_last_error.size());
+}
+
+int cubie_hip_compile(const char* source, const char* name, const char* options,
+                      void** code, size_t* code_size, char* log, size_t log_len) {
+    if (!source || !code || !code_size) return fail(CUBIE_E_INVALID, "null argument");
+    *code = nullptr;
+    *code_size = 0;
+    hiprtcProgram prog;
+    hiprtcResult rc = hiprtcCreateProgram(&prog, source, name ? name : "cubie_kernel.hip",
+                                          0, nullptr, nullptr);
+    if (rc != HIPRTC_SUCCESS)
+        return fail(CUBIE_E_COMPILE, std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(rc));
+    std::vector<std::string> opts = split_options(options);
+    std::vector<const char*> argv;
+    for (auto& o : opts) argv.push_back(o.c_str());
+    rc = hiprtcCompileProgram(prog, static_cast<int>(argv.size()), argv.data());
+    size_t lsz = 0;
+    hiprtcGetProgramLogSize(prog, &lsz);
+    std::string logs(lsz + 1, '\0');
+    if (lsz > 0) hiprtcGetProgramLog(prog, &logs[0]);
+    if (log && log_len > 0) {
+        std::strncpy(log, logs.c_str(), log_len - 1);
+        log[log_len - 1] = '\0';
+    }
+    if (rc != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail(CUBIE_E_COMPILE, std::string("hiprtcCompileProgram: ") +
+                                         hiprtcGetErrorString(rc) + "\n" + logs);
+    }
+    size_t csz = 0;
+    rc = hiprtcGetCodeSize(prog, &csz);
+    if (rc != HIPRTC_SUCCESS || csz == 0) {
+        hiprtcDestroyProgram(&prog);
+        return fail(CUBIE_E_COMPILE, "hiprtcGetCodeSize failed");
+    }
+    void* buf = std::malloc(csz);
+    if (!buf) {
+        hiprtcDestroyProgram(&prog);
+        return fail(CUBIE_E_INVALID, "out of host memory");
+    }
+    rc = hiprtcGetCode(prog, static_cast<char*>(buf));
+    hiprtcDestroyProgram(&prog);
+    if (rc != HIPRTC_SUCCESS) {
+        std::free(buf);
+        return fail(CUBIE_E_COMPILE, "hiprtcGetCode failed");
+    }
+    *code = buf;
+    *code_size = csz;
+    return CUBIE_OK;
+}
+
+void cubie_hip_free_code(void* code) { std::free(code); }
+
+int cubie_hip_load(const void* code, size_t code_size, const char* kernel_name,
+                   cubie_module_t* module) {
+    if (!code || code_size == 0 || !kernel_name || !module)
+        return fail(CUBIE_E_INVALID, "null argument");
+    auto* m = new cubie_module();
+    hipError_t err = hipModuleLoadData(&m->module, code);
+    if (err != hipSuccess) {
+        delete m;
+        return hip_fail(err, "hipModuleLoadData");
+    }
+    err = hipModuleGetFunction(&m->function, m->module, kernel_name);
+    if (err != hipSuccess) {
+        hipModuleUnload(m->module);
+        delete m;
+        return hip_fail(err, "hipModuleGetFunction");
+    }
+    m->kernel_name = kernel_name;
+    *module = m;
+    return CUBIE_OK;
+}
+
+int cubie_hip_unload(cubie_module_t module) {
+    if (!module) return CUBIE_OK;
+    hipError_t err = hipModuleUnload(module->module);
+    delete module;
+    if (err != hipSuccess) return hip_fail(err, "hipModuleUnload");
+    return CUBIE_OK;
+}
+
+int cubie_hip_kernel_info(cubie_module_t module, int* num_regs, int* shared_bytes,
+                          int* private_bytes, int* max_threads) {
+    if (!module) return fail(CUBIE_E_INVALID, "null module");
+    hipError_t err;
+    int v = 0;
+    if (num_regs) {
+        err = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, module->function);
+        if (err != hipSuccess) return hip_fail(err, "hipFuncGetAttribute(NUM_REGS)");
+        *num_regs = v;
+    }
+    if (shared_bytes) {
+        err = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, module->function);
+        if (err != hipSuccess) return hip_fail(err, "hipFuncGetAttribute(SHARED)");
+        *shared_bytes = v;
+    }
+    if (private_bytes) {
+        err = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, module->function);
+        if (err != hipSuccess) return hip_fail(err, "hipFuncGetAttribute(LOCAL)");
+        *private_bytes = v;
+    }
+    if (max_threads) {
+        err = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, module->function);
+        if (err != hipSuccess) return hip_fail(err, "hipFuncGetAttribute(MAX_THREADS)");
+        *max_threads = v;
+    }
+    return CUBIE_OK;
+}
+
+static int launch_impl(cubie_module_t module, const cubie_solve_args* args,
+                       unsigned grid, unsigned block, unsigned dynamic_lds,
+                       hipStream_t stream) {
+    if (!module || !args) return fail(CUBIE_E_INVALID, "null argument");
+    if (grid == 0 || block == 0 || block > 1024)
+        return fail(CUBIE_E_INVALID, "invalid launch shape");
+    cubie_solve_args local = *args;
+    void* params[] = {&local};
+    hipError_t err = hipModuleLaunchKernel(module->function, grid, 1, 1, block, 1, 1,
+                                           dynamic_lds, stream, params, nullptr);
+    if (err != hipSuccess) return hip_fail(err, "hipModuleLaunchKernel");
+    return CUBIE_OK;
+}
+
+int cubie_hip_launch(cubie_module_t module, const cubie_solve_args* args,
+                     unsigned grid, unsigned block, unsigned dynamic_lds, void* stream) {
+    return launch_impl(module, args, grid, block, dynamic_lds,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+int cubie_hip_launch_timed(cubie_module_t module, const cubie_solve_args* args,
+                           unsigned grid, unsigned block, unsigned dynamic_lds,
+                           void* stream, float* elapsed_ms) {
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t start, stop;
+    hipError_t err = hipEventCreate(&start);
+    if (err != hipSuccess) return hip_fail(err, "hipEventCreate");
+    err = hipEventCreate(&stop);
+    if (err != hipSuccess) {
+        hipEventDestroy(start);
+        return hip_fail(err, "hipEventCreate");
+    }
+    int rc = CUBIE_OK;
+    err = hipEventRecord(start, s);
+    if (err != hipSuccess) rc = hip_fail(err, "hipEventRecord");
+    if (rc == CUBIE_OK) rc = launch_impl(module, args, grid, block, dynamic_lds, s);
+    if (rc == CUBIE_OK) {
+        err = hipEventRecord(stop, s);
+        if (err != hipSuccess) rc = hip_fail(err, "hipEventRecord");
+    }
+    if (rc == CUBIE_OK) {
+        err = hipEventSynchronize(stop);
+        if (err != hipSuccess) rc = hip_fail(err, "hipEventSynchronize");
+    }
+    if (rc == CUBIE_OK && elapsed_ms) {
+        err = hipEventElapsedTime(elapsed_ms, start, stop);
+        if (err != hipSuccess) rc = hip_fail(err, "hipEventElapsedTime");
+    }
+    hipEventDestroy(start);
+    hipEventDestroy(stop);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// Device-side primitives shared by every integration kernel.
+//
+// Compiled by hiprtc as part of the generated translation unit; the
+// configuration struct `Cfg` and the system struct `System` precede this
+// file. Everything is header-only and force-inlined so that the whole loop
+// becomes one kernel with its per-system buffers in VGPRs.
+#pragma once
+
+#ifndef __forceinline__
+#define __forceinline__ inline __attribute__((always_inline))
+#endif
+
+namespace cubie {
+
+// Status bit vocabulary, result_codes.py:22-58 (CUBIE_RESULT_CODES).
+enum : int {
+    ST_SUCCESS = 0,
+    ST_NEWTON_MAX_ITERS = 2,
+    ST_LINEAR_MAX_ITERS = 4,
+    ST_STEP_TOO_SMALL = 8,
+    ST_STAGNATION = 64,
+    ST_BICGSTAB_BREAKDOWN = 128,
+    ST_NEWTON_DIVERGENCE = 256,
+};
+
+// Python's max()/min() selection order (the reference's device code calls
+// them on floats): max(a, b) keeps a unless b > a.
+template <typename T>
+__device__ __forceinline__ T pmax(T a, T b) { return (b > a) ? b : a; }
+template <typename T>
+__device__ __forceinline__ T pmin(T a, T b) { return (b < a) ? b : a; }
+// _utils.py:315-342 clamp(value, lo, hi) = max(lo, min(value, hi))
+template <typename T>
+__device__ __forceinline__ T clampv(T v, T lo, T hi) { return pmax(lo, pmin(v, hi)); }
+
+__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dpow(double x, double e) { return pow(x, e); }
+__device__ __forceinline__ float dpow(float x, float e) { return powf(x, e); }
+__device__ __forceinline__ bool dfinite(double x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ bool dfinite(float x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ double dfloor(double x) { return __builtin_floor(x); }
+__device__ __forceinline__ float dfloor(float x) { return __builtin_floorf(x); }
+
+// Wavefront-wide vote over the currently active lanes, the wave64
+// counterpart of the reference's all_sync(activemask(), pred)
+// (cuda_simsafe.py:610).
+__device__ __forceinline__ bool wave_all(bool pred) { return __all(pred ? 1 : 0) != 0; }
+
+// Fixed-size vector kept in registers when indexed with constants.
+template <typename T, int N>
+struct Vec {
+    T v[N > 0 ? N : 1];
+    __device__ __forceinline__ T& operator[](int i) { return v[i]; }
+    __device__ __forceinline__ const T& operator[](int i) const { return v[i]; }
+    __device__ __forceinline__ T* data() { return v; }
+    __device__ __forceinline__ const T* data() const { return v; }
+};
+
+// Strided view of one run's column in a (..., n_runs) array.
+template <typename T>
+struct Column {
+    T* base;
+    long long stride;
+    __device__ __forceinline__ T& operator[](long long i) const { return base[i * stride]; }
+};
+
+}  // namespace cubie

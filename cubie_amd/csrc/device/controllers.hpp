@@ -1,0 +1,115 @@
+// Step-size controllers (fixed, I, PI, PID, Gustafsson).
+//
+// Each follows the reference device function of the same name:
+//   I          integrators/step_control/adaptive_I_controller.py (controller_I)
+//   PI         integrators/step_control/adaptive_PI_controller.py:178-268
+//   PID        integrators/step_control/adaptive_PID_controller.py:184-284
+//   Gustafsson integrators/step_control/gustafsson_controller.py:173-285
+// The weighted mean-square norm, the 1e16 stand-in for non-finite norms, the
+// deadband, the reject cap min(gain, safety), the truncated-step freeze and
+// the history commit rule are the reference's; the arithmetic order is kept
+// so the FP64 kernel reproduces the oracle step for step.
+//
+// The norm and the accept decision can run in a wider type (NormT) than the
+// stage arithmetic: the compensated FP32 mode instantiates NormT = double.
+#pragma once
+
+namespace cubie {
+
+enum { CTRL_FIXED = 0, CTRL_I = 1, CTRL_PI = 2, CTRL_PID = 3, CTRL_GUSTAFSSON = 4 };
+
+template <typename Cfg>
+struct Controller {
+    using real = typename Cfg::real;
+    using nreal = typename Cfg::norm_real;   // precision of norm + decision
+    nreal history[2];                        // err_prev / err_prev_prev or dt_prev / err_prev
+
+    __device__ __forceinline__ void init() { history[0] = nreal(0); history[1] = nreal(0); }
+
+    // Weighted mean-square error norm (no square root), with an optional
+    // per-run relative tolerance that replaces rtol[i] when supplied.
+    template <typename A, typename B, typename E>
+    __device__ __forceinline__ nreal norm(const A& state, const B& state_prev, const E& error,
+                                          nreal rtol_run) const {
+        nreal nrm2 = nreal(0);
+#pragma unroll
+        for (int i = 0; i < Cfg::n; ++i) {
+            nreal ei = pmax(dabs(nreal(error[i])), nreal(1e-16));
+            nreal rt = Cfg::per_run_rtol ? rtol_run : nreal(Cfg::rtol[i]);
+            nreal tol = nreal(Cfg::atol[i]) +
+                        rt * pmax(dabs(nreal(state[i])), dabs(nreal(state_prev[i])));
+            nreal ratio = ei / tol;
+            nrm2 += ratio * ratio;
+        }
+        nrm2 = nrm2 * nreal(Cfg::inv_n);
+        nrm2 = dfinite(nrm2) ? nrm2 : nreal(1e16);
+        return nrm2;
+    }
+
+    __device__ __forceinline__ nreal deadband(nreal gain) const {
+        if constexpr (!Cfg::deadband_disabled) {
+            bool within = (gain >= nreal(Cfg::deadband_min)) && (gain <= nreal(Cfg::deadband_max));
+            gain = within ? nreal(1) : gain;
+        }
+        return gain;
+    }
+
+    // Updates dt in place and returns the status bits; accept_out is the
+    // controller's verdict before the loop folds in step failure.
+    template <typename A, typename B, typename E>
+    __device__ __forceinline__ int control(real& dt, const A& state, const B& state_prev,
+                                           const E& error, int niters, bool truncated,
+                                           bool& accept_out, nreal rtol_run) {
+        const nreal nrm2 = norm(state, state_prev, error, rtol_run);
+        const bool accept = nrm2 <= nreal(1);
+        accept_out = accept;
+        const bool freeze = accept && truncated;
+        const bool commit = accept && !truncated;
+        const nreal safety = nreal(Cfg::safety);
+        const nreal cur = nreal(dt);
+        nreal gain;
+        if constexpr (Cfg::controller == CTRL_I) {
+            nreal gaintmp = safety * dpow(nrm2, -nreal(Cfg::order_exponent));
+            gain = clampv(gaintmp, nreal(Cfg::min_gain), nreal(Cfg::max_gain));
+        } else if constexpr (Cfg::controller == CTRL_PI) {
+            nreal err_prev = history[0];
+            nreal pgain = dpow(nrm2, -nreal(Cfg::kp_exp));
+            nreal err_source = err_prev > nreal(0) ? err_prev : nrm2;
+            nreal igain = dpow(err_source, -nreal(Cfg::ki_exp));
+            nreal gain_new = safety * pgain * igain;
+            gain = clampv(gain_new, nreal(Cfg::min_gain), nreal(Cfg::max_gain));
+            history[0] = commit ? nrm2 : err_prev;
+        } else if constexpr (Cfg::controller == CTRL_PID) {
+            nreal err_prev = history[0], err_prev_prev = history[1];
+            nreal eps = err_prev > nreal(0) ? err_prev : nrm2;
+            nreal epps = err_prev_prev > nreal(0) ? err_prev_prev : eps;
+            nreal gain_new = safety * dpow(nrm2, -nreal(Cfg::kp_exp)) *
+                             dpow(eps, -nreal(Cfg::ki_exp)) * dpow(epps, -nreal(Cfg::kd_exp));
+            gain = clampv(gain_new, nreal(Cfg::min_gain), nreal(Cfg::max_gain));
+            history[1] = commit ? err_prev : err_prev_prev;
+            history[0] = commit ? nrm2 : err_prev;
+        } else {  // Gustafsson
+            nreal dt_prev = pmax(history[0], nreal(1e-16));
+            nreal err_prev = pmax(history[1], nreal(1e-16));
+            nreal denom = nreal(niters + 2 * Cfg::newton_target_iters);
+            nreal tmp = nreal(Cfg::gain_numerator) / denom;
+            nreal fac = safety < tmp ? safety : tmp;
+            nreal expo = nreal(Cfg::order_exponent);
+            nreal gain_basic = fac * dpow(nrm2, -expo);
+            nreal ratio = nrm2 * nrm2 / err_prev;
+            nreal gain_gus = safety * (cur / dt_prev) * dpow(ratio, -expo);
+            gain = gain_gus < gain_basic ? gain_gus : gain_basic;
+            gain = (accept && dt_prev > nreal(1e-16)) ? gain : gain_basic;
+            gain = clampv(gain, nreal(Cfg::min_gain), nreal(Cfg::max_gain));
+            history[0] = commit ? cur : history[0];
+            history[1] = commit ? nrm2 : history[1];
+        }
+        gain = deadband(gain);
+        gain = accept ? gain : pmin(gain, safety);
+        const real dt_new_raw = real(dt * real(gain));
+        dt = freeze ? dt : clampv(dt_new_raw, real(Cfg::dt_min), real(Cfg::dt_max));
+        return (freeze || dt_new_raw > real(Cfg::dt_min)) ? ST_SUCCESS : ST_STEP_TOO_SMALL;
+    }
+};
+
+}  // namespace cubie

@@ -1,0 +1,77 @@
+// Driver-array evaluation: piecewise polynomials in Horner form.
+//
+// Behaviour follows the reference's evaluate_all / evaluate_time_derivative
+// (array_interpolator.py:463-545): segment index from floor((t - start) /
+// period), wrap or clamp, Horner over `order + 1` coefficients laid out as
+// [segment][driver][power]. The coefficient table is read through the
+// read-only cache; all lanes of a wave usually hit the same or adjacent
+// segments, so the gather stays in L1/L2.
+#pragma once
+
+namespace cubie {
+
+template <typename Cfg>
+struct Drivers {
+    using real = typename Cfg::real;
+    const real* __restrict__ coeffs;
+
+    __device__ __forceinline__ void locate(real time, int& seg, real& tau, bool& in_range) const {
+        constexpr real inv_res = Cfg::drv_inv_resolution;
+        constexpr real start = Cfg::drv_evaluation_start;
+        constexpr int nseg = Cfg::drv_segments;
+        real scaled = (time - start) * inv_res;
+        real scaled_floor = dfloor(scaled);
+        int idx = int(scaled_floor);
+        if constexpr (Cfg::drv_wrap) {
+            seg = idx % nseg;
+            if (seg < 0) seg += nseg;
+            tau = real(scaled - scaled_floor);
+            in_range = true;
+        } else {
+            in_range = (scaled >= real(0)) && (double(scaled) <= double(nseg));
+            seg = idx < 0 ? 0 : idx;
+            seg = seg >= nseg ? nseg - 1 : seg;
+            tau = real(scaled - real(seg));
+        }
+    }
+
+    __device__ __forceinline__ void eval(real time, real* out) const {
+        if constexpr (Cfg::n_drivers > 0) {
+            int seg;
+            real tau;
+            bool in_range;
+            locate(time, seg, tau, in_range);
+            constexpr int K = Cfg::drv_order + 1;
+            const real* row = coeffs + (long long)seg * (Cfg::n_drivers * K);
+#pragma unroll
+            for (int i = 0; i < Cfg::n_drivers; ++i) {
+                real acc = real(0);
+#pragma unroll
+                for (int k = Cfg::drv_order; k >= 0; --k) acc = acc * tau + __ldg(&row[i * K + k]);
+                out[i] = in_range ? acc : real(0);
+            }
+        }
+    }
+
+    // d/dt of every driver polynomial (array_interpolator.py:511-545).
+    __device__ __forceinline__ void eval_dt(real time, real* out) const {
+        if constexpr (Cfg::n_drivers > 0) {
+            int seg;
+            real tau;
+            bool in_range;
+            locate(time, seg, tau, in_range);
+            constexpr int K = Cfg::drv_order + 1;
+            const real* row = coeffs + (long long)seg * (Cfg::n_drivers * K);
+#pragma unroll
+            for (int i = 0; i < Cfg::n_drivers; ++i) {
+                real acc = real(0);
+#pragma unroll
+                for (int k = Cfg::drv_order; k > 0; --k)
+                    acc = acc * tau + real(k) * __ldg(&row[i * K + k]);
+                out[i] = in_range ? acc * Cfg::drv_inv_resolution : real(0);
+            }
+        }
+    }
+};
+
+}  // namespace cubie

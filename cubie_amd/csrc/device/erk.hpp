@@ -1,0 +1,112 @@
+// Tableau-driven explicit Runge-Kutta step.
+//
+// Same arithmetic as the reference's streamed-accumulator ERK step
+// (integrators/algorithms/generic_erk.py:359-564): stage states are built as
+// (sum_j a_ij k_j) * dt + y with the sums accumulated in stage order, the
+// solution and error as (sum b_i k_i) * dt + y and (sum d_i k_i) * dt with
+// d = b - b_hat, and direct copies when b (or b_hat) equals a row of A. The
+// tableau is a compile-time constant, so zero coefficients cost nothing and
+// every stage vector lives in VGPRs. First-same-as-last reuse is taken only
+// when every active lane of the wave accepted its previous step, as the
+// reference does with its warp vote.
+#pragma once
+
+namespace cubie {
+
+template <typename Cfg, typename Sys>
+struct ERKStep {
+    using real = typename Cfg::real;
+    static constexpr int n = Cfg::n;
+    static constexpr int s = Cfg::stages;
+    Vec<real, n> stage_rhs;   // persistent: FSAL cache
+
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int i = 0; i < n; ++i) stage_rhs[i] = real(0);
+    }
+
+    template <typename Ctx>
+    __device__ __forceinline__ int step(Ctx& ctx, const Vec<real, n>& state, Vec<real, n>& proposed,
+                                        Vec<real, Cfg::n_error>& error, real dt, real time,
+                                        double /*t64*/, bool first_step, bool prev_accepted,
+                                        int* counters) {
+        const real current_time = time;
+        const real end_time = current_time + dt;
+        Vec<real, (s > 1 ? (s - 1) * n : 1)> acc;
+
+        bool use_cached = false;
+        if constexpr (Cfg::fsal && s > 1) {
+            if (!first_step) use_cached = wave_all(prev_accepted);
+        }
+        if (!use_cached || s == 1) {
+            Sys::dxdt(state.data(), ctx.params.data(), ctx.drivers.data(), ctx.observables.data(),
+                      stage_rhs.data(), current_time);
+        }
+        // Accumulators for the solution and the error estimate.
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            if constexpr (Cfg::accumulates_output) proposed[i] = real(0) + Cfg::b[0] * stage_rhs[i];
+            if constexpr (Cfg::has_error && Cfg::accumulates_error)
+                error[i] = real(0) + Cfg::d[0] * stage_rhs[i];
+        }
+#pragma unroll
+        for (int i = 0; i < (s > 1 ? (s - 1) * n : 1); ++i) acc[i] = real(0);
+#pragma unroll
+        for (int prev = 0; prev < s - 1; ++prev) {
+            const int stage_idx = prev + 1;
+            // Stream stage `prev` into every later stage's accumulator. Zero
+            // coefficients are skipped: adding +-0 to the running sum is exact.
+#pragma unroll
+            for (int succ = prev; succ < s - 1; ++succ) {
+                if (Cfg::A[succ + 1][prev] != 0.0) {
+#pragma unroll
+                    for (int i = 0; i < n; ++i) acc[succ * n + i] += Cfg::A[succ + 1][prev] * stage_rhs[i];
+                }
+            }
+            const real stage_time = current_time + dt * Cfg::c[stage_idx];
+            // Stage state y_j = (sum_i a_ji k_i) * dt + y
+#pragma unroll
+            for (int i = 0; i < n; ++i) acc[prev * n + i] = acc[prev * n + i] * dt + state[i];
+            ctx.drv.eval(stage_time, ctx.proposed_drivers.data());
+            if constexpr (Cfg::n_obs > 0)
+                Sys::observables(&acc[prev * n], ctx.params.data(), ctx.proposed_drivers.data(),
+                                 ctx.proposed_observables.data(), stage_time);
+            Sys::dxdt(&acc[prev * n], ctx.params.data(), ctx.proposed_drivers.data(),
+                      ctx.proposed_observables.data(), stage_rhs.data(), stage_time);
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                if constexpr (Cfg::accumulates_output) {
+                    if (Cfg::b[stage_idx] != 0.0) proposed[i] += Cfg::b[stage_idx] * stage_rhs[i];
+                }
+                if constexpr (Cfg::has_error && Cfg::accumulates_error) {
+                    if (Cfg::d[stage_idx] != 0.0) error[i] += Cfg::d[stage_idx] * stage_rhs[i];
+                }
+            }
+        }
+        if constexpr (Cfg::b_row >= 0) {
+#pragma unroll
+            for (int i = 0; i < n; ++i) proposed[i] = acc[(Cfg::b_row - 1) * n + i];
+        }
+        if constexpr (Cfg::has_error && Cfg::b_hat_row >= 0) {
+#pragma unroll
+            for (int i = 0; i < n; ++i) error[i] = acc[(Cfg::b_hat_row - 1) * n + i];
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            if constexpr (Cfg::accumulates_output) proposed[i] = proposed[i] * dt + state[i];
+            if constexpr (Cfg::has_error) {
+                if constexpr (Cfg::accumulates_error) error[i] *= dt;
+                else error[i] = proposed[i] - error[i];
+            }
+        }
+        ctx.drv.eval(end_time, ctx.proposed_drivers.data());
+        if constexpr (Cfg::n_obs > 0)
+            Sys::observables(proposed.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                             ctx.proposed_observables.data(), end_time);
+        counters[0] = 0;
+        counters[1] = 0;
+        return ST_SUCCESS;
+    }
+};
+
+}  // namespace cubie

@@ -1,0 +1,281 @@
+// The per-system integration loop and the kernel entry points.
+//
+// One lane integrates one system from t0 to t0 + settling + duration. The
+// control flow is the reference's loop_fn (integrators/loops/ode_loop.py:
+// 562-984): event tests against the save / summary schedules and their stop
+// times, the positive-gap clamp onto the next output event, the step, the
+// failed-step 1e16 error, the controller, the counters, the two-stall
+// stagnation rule, the status fold on an irrecoverable end, the predicated
+// commit of t / state / drivers / observables and finally the saves and
+// summary updates. Time is accumulated in double; everything else is in
+// Cfg::real (or, in compensated mode, real plus a running correction term).
+//
+// The loop is split into begin() / advance() / status so that two kernels can
+// share it:
+//   cubie_integrate            - one lane per system, grid covers the batch
+//   cubie_integrate_persistent - a grid sized to fill the GPU; whenever a lane
+//                                finishes its system it takes the next one
+//                                from an atomic counter (one wave-aggregated
+//                                atomic per refill), so lanes that finish
+//                                early keep working on uneven batches.
+#pragma once
+
+namespace cubie {
+
+template <typename Cfg>
+struct LoopCtx {
+    using real = typename Cfg::real;
+    Vec<real, Cfg::n_params> params;
+    Vec<real, Cfg::n_drivers> drivers;
+    Vec<real, Cfg::n_drivers> proposed_drivers;
+    Vec<real, Cfg::n_obs> observables;
+    Vec<real, Cfg::n_obs> proposed_observables;
+    Drivers<Cfg> drv;
+};
+
+template <typename Cfg, typename Sys, typename Stepper>
+struct Integrator {
+    using real = typename Cfg::real;
+    using nreal = typename Cfg::norm_real;
+    static constexpr int n = Cfg::n;
+    static constexpr bool summarise = Cfg::summarise_state || Cfg::summarise_obs;
+
+    LoopCtx<Cfg> ctx;
+    Stepper stepper;
+    Controller<Cfg> ctrl;
+    Summaries<Cfg> summ;
+    Vec<real, n> state, proposal;
+    Vec<real, Cfg::n_error> error;
+    long long run;
+    double t;
+    real t_prec, t_end, save_stop, summary_stop, next_save, next_update, dt, dt_raw;
+    nreal rtol_run;
+    int counters_since_save[4];
+    int proposed_counters[2];
+    int stagnant, update_idx, status, iteration_status;
+    long long save_idx, summary_idx;
+    bool first_step, prev_accepted, irrecoverable;
+
+    __device__ __forceinline__ void begin(const cubie_solve_args& a, long long r) {
+        run = r;
+        const long long N = a.n_runs;
+        ctx.drv.coeffs = static_cast<const real*>(a.driver_coeffs);
+        stepper.init();
+        ctrl.init();
+        summ.zero();
+        const real duration = a.duration_per_run ? real(a.duration_per_run[run]) : real(a.duration);
+        const real settling = real(a.warmup);
+        const real t0 = real(a.t0);
+        save_stop = real(a.save_stop);
+        summary_stop = real(a.summary_stop);
+        rtol_run = nreal(0);
+        if constexpr (Cfg::per_run_rtol) rtol_run = nreal(static_cast<const real*>(a.rtol_per_run)[run]);
+        t = double(t0);
+        t_prec = real(t);
+        t_end = real(settling + t0 + duration);
+#pragma unroll
+        for (int i = 0; i < Cfg::n_error; ++i) error[i] = real(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) counters_since_save[k] = 0;
+        proposed_counters[0] = proposed_counters[1] = 0;
+        first_step = true;
+        prev_accepted = true;
+        stagnant = 0;
+        save_idx = 0;
+        summary_idx = 0;
+        update_idx = 0;
+        next_save = real(settling + t0);
+        next_update = real(settling + t0);
+        const real* inits = static_cast<const real*>(a.inits);
+        const real* params = static_cast<const real*>(a.params);
+#pragma unroll
+        for (int i = 0; i < n; ++i) state[i] = inits[i * N + run];
+#pragma unroll
+        for (int i = 0; i < Cfg::n_params; ++i) ctx.params[i] = params[i * N + run];
+#pragma unroll
+        for (int i = 0; i < Cfg::n_drivers; ++i) { ctx.drivers[i] = real(0); ctx.proposed_drivers[i] = real(0); }
+#pragma unroll
+        for (int i = 0; i < Cfg::n_obs; ++i) { ctx.observables[i] = real(0); ctx.proposed_observables[i] = real(0); }
+#pragma unroll
+        for (int i = 0; i < n; ++i) proposal[i] = real(0);
+        if constexpr (Cfg::n_drivers > 0) ctx.drv.eval(t_prec, ctx.drivers.data());
+        if constexpr (Cfg::n_obs > 0)
+            Sys::observables(state.data(), ctx.params.data(), ctx.drivers.data(),
+                             ctx.observables.data(), t_prec);
+        if (settling == real(0)) {
+            if constexpr (Cfg::save_regularly) next_save = real(next_save + Cfg::save_every);
+            if constexpr (Cfg::summarise_regularly) next_update = real(Cfg::sample_summaries_every + next_update);
+            save_state<Cfg>(a, run, save_idx, state, ctx.observables, counters_since_save, t_prec);
+            save_idx += 1;
+            if constexpr (summarise) summ.save(a, run, summary_idx);  // resets the window buffers
+        }
+        status = ST_SUCCESS;
+        iteration_status = 0;
+        dt = Cfg::dt0;
+        dt_raw = dt;
+        irrecoverable = false;
+    }
+
+    // One attempted step. Returns true once the system is finished.
+    __device__ __forceinline__ bool advance(const cubie_solve_args& a) {
+        const real end_of_step = t_prec + dt_raw;
+        bool finished;
+        bool save_finished = false, summary_finished = false, at_end = false;
+        if constexpr (Cfg::save_regularly || Cfg::summarise_regularly) {
+            finished = true;
+            if constexpr (Cfg::save_regularly) {
+                save_finished = next_save > save_stop;
+                finished = finished && save_finished;
+            }
+            if constexpr (Cfg::summarise_regularly) {
+                summary_finished = next_update > summary_stop;
+                finished = finished && summary_finished;
+            }
+        } else {
+            finished = end_of_step >= t_end;
+        }
+        if constexpr (Cfg::save_last) {
+            at_end = (t_prec < t_end) && finished;
+            finished = finished && !at_end;
+        }
+        finished = finished || irrecoverable;
+        if (finished) return true;
+
+        bool do_save = false, do_update = false;
+        if constexpr (Cfg::save_regularly) do_save = (end_of_step >= next_save) && !save_finished;
+        if constexpr (Cfg::summarise_regularly) do_update = (end_of_step >= next_update) && !summary_finished;
+        if constexpr (Cfg::save_last) do_save = do_save || at_end;
+
+        real dt_eff = dt_raw;
+        bool truncated = false;
+        if (do_save || do_update) {
+            real next_event = t_end;
+            if constexpr (Cfg::save_regularly) { if (do_save) next_event = pmin(next_event, next_save); }
+            if constexpr (Cfg::summarise_regularly) { if (do_update) next_event = pmin(next_event, next_update); }
+            const real gap = next_event - t_prec;
+            dt_eff = gap > real(0) ? gap : dt_raw;
+            truncated = dt_eff != dt_raw;
+        }
+
+        const int step_status = stepper.step(ctx, state, proposal, error, dt_eff, t_prec, t,
+                                             first_step, prev_accepted, proposed_counters);
+        first_step = false;
+        const int niters = proposed_counters[0];
+        iteration_status |= step_status;
+        const bool step_failed = step_status != 0;
+        if constexpr (!Cfg::adaptive) irrecoverable = irrecoverable || step_failed;
+#pragma unroll
+        for (int i = 0; i < Cfg::n_error; ++i) error[i] = step_failed ? real(1e16) : error[i];
+
+        bool accept;
+        if constexpr (Cfg::adaptive) {
+            bool ctrl_accept = false;
+            const int cs = ctrl.control(dt, proposal, state, error, niters, truncated, ctrl_accept, rtol_run);
+            accept = ctrl_accept && !step_failed;
+            iteration_status |= cs;
+            irrecoverable = irrecoverable || ((cs & ST_STEP_TOO_SMALL) != 0);
+        } else {
+            accept = !step_failed;
+        }
+        dt_raw = dt;
+
+        if constexpr (Cfg::save_counters) {
+            counters_since_save[0] += proposed_counters[0];
+            counters_since_save[1] += proposed_counters[1];
+            counters_since_save[2] += 1;
+            counters_since_save[3] += accept ? 0 : 1;
+        }
+
+        double t_proposal = t + double(dt_eff);
+        if constexpr (Cfg::compensated) t_proposal = stepper.time_after(t, dt_eff, accept);
+        stagnant = (t_proposal == t) ? stagnant + 1 : 0;
+        const bool is_stagnant = stagnant >= 2;
+        if (is_stagnant) iteration_status |= ST_STAGNATION;
+        irrecoverable = irrecoverable || is_stagnant;
+        if (irrecoverable) status |= iteration_status;
+        if (accept) iteration_status = 0;
+
+        t = accept ? t_proposal : t;
+        t_prec = real(t);
+        if constexpr (Cfg::compensated) stepper.commit(accept);
+#pragma unroll
+        for (int i = 0; i < n; ++i) state[i] = accept ? proposal[i] : state[i];
+#pragma unroll
+        for (int i = 0; i < Cfg::n_drivers; ++i)
+            ctx.drivers[i] = accept ? ctx.proposed_drivers[i] : ctx.drivers[i];
+#pragma unroll
+        for (int i = 0; i < Cfg::n_obs; ++i)
+            ctx.observables[i] = accept ? ctx.proposed_observables[i] : ctx.observables[i];
+        prev_accepted = accept;
+
+        do_save = do_save && accept;
+        do_update = do_update && accept;
+        if (do_save) {
+            if constexpr (Cfg::save_regularly) next_save += Cfg::save_every;
+            save_state<Cfg>(a, run, save_idx, state, ctx.observables, counters_since_save, t_prec);
+            save_idx += 1;
+            if constexpr (Cfg::save_counters) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) counters_since_save[k] = 0;
+            }
+        }
+        if (do_update) {
+            if constexpr (Cfg::summarise_regularly) next_update += Cfg::sample_summaries_every;
+            if constexpr (summarise) {
+                summ.update(state, ctx.observables, update_idx);
+                update_idx += 1;
+                if (update_idx % Cfg::samples_per_summary == 0) {
+                    summ.save(a, run, summary_idx);
+                    summary_idx += 1;
+                }
+            }
+        }
+        return false;
+    }
+};
+
+template <typename Cfg, typename Sys, typename Stepper>
+__device__ __forceinline__ void integrate_plain(const cubie_solve_args& a) {
+    const long long run = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (run >= a.n_runs) return;
+    Integrator<Cfg, Sys, Stepper> it;
+    it.begin(a, run);
+    while (!it.advance(a)) {
+    }
+    a.status_out[run] = it.status;
+}
+
+// Wave-aggregated ticket: every lane with want=true receives a distinct run
+// index; one atomic per wave per refill.
+__device__ __forceinline__ long long grab_runs(int* counter, bool want) {
+    const unsigned long long mask = __ballot(want ? 1 : 0);
+    const int count = __popcll(mask);
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (count > 0 && lane == leader) base = atomicAdd(counter, count);
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    const unsigned long long lower = mask & ((1ull << lane) - 1ull);
+    return want ? (long long)base + __popcll(lower) : -1;
+}
+
+template <typename Cfg, typename Sys, typename Stepper>
+__device__ __forceinline__ void integrate_persistent(const cubie_solve_args& a) {
+    Integrator<Cfg, Sys, Stepper> it;
+    long long run = grab_runs(a.work_counter, true);
+    bool active = run < a.n_runs;
+    if (active) it.begin(a, run);
+    while (__any(active ? 1 : 0)) {
+        bool done = false;
+        if (active) done = it.advance(a);
+        if (active && done) a.status_out[it.run] = it.status;
+        // Lanes that just finished take the next system of the batch.
+        const long long next = grab_runs(a.work_counter, active && done);
+        if (active && done) {
+            active = next < a.n_runs;
+            if (active) it.begin(a, next);
+        }
+    }
+}
+
+}  // namespace cubie

@@ -1,0 +1,316 @@
+// Saved-state writes and summary metrics.
+//
+// Saves follow the reference's save_state_func (outputhandling/save_state.py):
+// selected states, then the time in the column after them, selected
+// observables, then the four iteration counters. Summary metrics follow
+// outputhandling/summarymetrics/*.py update/save bodies, chained in request
+// order over a per-variable buffer of Cfg::summ_buf_height slots
+// (update_summaries.py:169-233, save_summaries.py:189-334).
+//
+// Every store goes to [row][column][run] with the run index fastest, so a
+// wave that saves on the same loop iteration writes whole cache lines.
+// Stores beyond the allocated rows are dropped (the host sizes the rows so
+// this never triggers in a well-formed run).
+#pragma once
+
+namespace cubie {
+
+enum {
+    M_MEAN = 0, M_MAX = 1, M_RMS = 2, M_PEAKS = 3, M_STD = 4, M_MIN = 5,
+    M_MAX_MAGNITUDE = 6, M_EXTREMA = 7, M_NEGATIVE_PEAKS = 8,
+    M_MEAN_STD_RMS = 9, M_MEAN_STD = 10, M_STD_RMS = 11,
+    M_DXDT_MAX = 12, M_DXDT_MIN = 13, M_DXDT_EXTREMA = 14,
+    M_D2XDT2_MAX = 15, M_D2XDT2_MIN = 16, M_D2XDT2_EXTREMA = 17,
+    M_SPIKE_COUNT = 18,
+};
+
+template <typename real>
+__device__ __forceinline__ void metric_update(int id, int param, real value, real* buf, int idx) {
+    switch (id) {
+    case M_MEAN: buf[0] += value; break;
+    case M_MAX: if (value > buf[0]) buf[0] = value; break;
+    case M_MIN: if (value < buf[0]) buf[0] = value; break;
+    case M_RMS: {
+        real ss = buf[0];
+        if (idx == 0) ss = real(0);
+        ss += value * value;
+        buf[0] = ss;
+        break;
+    }
+    case M_PEAKS:
+    case M_NEGATIVE_PEAKS: {
+        real prev = buf[0], prev_prev = buf[1];
+        int counter = int(buf[2]);
+        if (idx >= 2 && counter < param && prev_prev != real(0)) {
+            bool hit = (id == M_PEAKS) ? (prev > value && prev_prev < prev)
+                                       : (prev < value && prev_prev > prev);
+            if (hit) {
+#pragma unroll
+                for (int p = 0; p < 64; ++p) {
+                    if (p >= param) break;
+                    if (p == counter) buf[3 + p] = real(idx - 1);
+                }
+                buf[2] = real(int(buf[2]) + 1);
+            }
+        }
+        buf[0] = value;
+        buf[1] = prev;
+        break;
+    }
+    case M_STD: case M_MEAN_STD: case M_MEAN_STD_RMS: case M_STD_RMS: {
+        if (idx == 0) { buf[0] = value; buf[1] = real(0); buf[2] = real(0); }
+        real sv = value - buf[0];
+        buf[1] += sv;
+        buf[2] += sv * sv;
+        break;
+    }
+    case M_MAX_MAGNITUDE: {
+        real a = dabs(value);
+        if (a > buf[0]) buf[0] = a;
+        break;
+    }
+    case M_EXTREMA:
+        if (value > buf[0]) buf[0] = value;
+        if (value < buf[1]) buf[1] = value;
+        break;
+    case M_DXDT_MAX: case M_DXDT_MIN: {
+        real du = value - buf[0];
+        bool upd = (id == M_DXDT_MAX ? du > buf[1] : du < buf[1]) && idx >= 1;
+        buf[1] = upd ? du : buf[1];
+        buf[0] = value;
+        break;
+    }
+    case M_DXDT_EXTREMA: {
+        real du = value - buf[0];
+        bool primed = idx >= 1;
+        bool umax = du > buf[1] && primed;
+        bool umin = du < buf[2] && primed;
+        buf[1] = umax ? du : buf[1];
+        buf[2] = umin ? du : buf[2];
+        buf[0] = value;
+        break;
+    }
+    case M_D2XDT2_MAX: case M_D2XDT2_MIN: {
+        real sd = value - real(2) * buf[0] + buf[1];
+        bool upd = (id == M_D2XDT2_MAX ? sd > buf[2] : sd < buf[2]) && idx >= 2;
+        buf[2] = upd ? sd : buf[2];
+        buf[1] = buf[0];
+        buf[0] = value;
+        break;
+    }
+    case M_D2XDT2_EXTREMA: {
+        real sd = value - real(2) * buf[0] + buf[1];
+        bool primed = idx >= 2;
+        bool umax = sd > buf[2] && primed;
+        bool umin = sd < buf[3] && primed;
+        buf[2] = umax ? sd : buf[2];
+        buf[3] = umin ? sd : buf[3];
+        buf[1] = buf[0];
+        buf[0] = value;
+        break;
+    }
+    case M_SPIKE_COUNT: {
+        // Upward crossings of the threshold `param` (buffer: prev, count).
+        const real threshold = real(param);
+        bool cross = idx >= 1 && buf[0] < threshold && value >= threshold;
+        buf[1] = cross ? buf[1] + real(1) : buf[1];
+        buf[0] = value;
+        break;
+    }
+    default: cubie_plugin_metric_update(id, param, value, buf, idx); break;
+    }
+}
+
+// Division by the int32 sample count follows Numba's promotion to float64
+// before the store narrows it (mean.py, rms.py, std.py, mean_std*.py).
+template <typename real>
+__device__ __forceinline__ void metric_save(int id, int param, real* buf, real* out, int spp,
+                                            real sample_every) {
+    switch (id) {
+    case M_MEAN:
+        out[0] = real(double(buf[0]) / double(spp));
+        buf[0] = real(0);
+        break;
+    case M_MAX: out[0] = buf[0]; buf[0] = real(-1.0e30); break;
+    case M_MIN: out[0] = buf[0]; buf[0] = real(1.0e30); break;
+    case M_RMS:
+        out[0] = real(__builtin_sqrt(double(buf[0]) / double(spp)));
+        buf[0] = real(0);
+        break;
+    case M_PEAKS: case M_NEGATIVE_PEAKS:
+        for (int p = 0; p < param; ++p) { out[p] = buf[3 + p]; buf[3 + p] = real(0); }
+        buf[2] = real(0);
+        break;
+    case M_STD: {
+        double ms = double(buf[1]) / double(spp);
+        double mss = double(buf[2]) / double(spp);
+        double var = mss - ms * ms;
+        out[0] = real(__builtin_sqrt(var));
+        buf[0] = real(double(buf[0]) + ms);
+        buf[1] = real(0);
+        buf[2] = real(0);
+        break;
+    }
+    case M_MEAN_STD: case M_MEAN_STD_RMS: case M_STD_RMS: {
+        double shift = double(buf[0]);
+        double ms = double(buf[1]) / double(spp);
+        double mss = double(buf[2]) / double(spp);
+        double mean = shift + ms;
+        double var = mss - ms * ms;
+        double sd = __builtin_sqrt(var);
+        double msq = mss + double(real(2) * buf[0]) * ms + double(buf[0] * buf[0]);
+        double rms = __builtin_sqrt(msq);
+        if (id == M_MEAN_STD) { out[0] = real(mean); out[1] = real(sd); }
+        else if (id == M_MEAN_STD_RMS) { out[0] = real(mean); out[1] = real(sd); out[2] = real(rms); }
+        else { out[0] = real(sd); out[1] = real(rms); }
+        buf[0] = real(mean);
+        buf[1] = real(0);
+        buf[2] = real(0);
+        break;
+    }
+    case M_MAX_MAGNITUDE: out[0] = buf[0]; buf[0] = real(0); break;
+    case M_EXTREMA:
+        out[0] = buf[0]; out[1] = buf[1];
+        buf[0] = real(-1.0e30); buf[1] = real(1.0e30);
+        break;
+    case M_DXDT_MAX: out[0] = buf[1] / sample_every; buf[1] = real(-1.0e30); break;
+    case M_DXDT_MIN: out[0] = buf[1] / sample_every; buf[1] = real(1.0e30); break;
+    case M_DXDT_EXTREMA:
+        out[0] = buf[1] / sample_every; out[1] = buf[2] / sample_every;
+        buf[1] = real(-1.0e30); buf[2] = real(1.0e30);
+        break;
+    case M_D2XDT2_MAX: case M_D2XDT2_MIN: {
+        real sq = sample_every * sample_every;
+        out[0] = buf[2] / sq;
+        buf[2] = real(id == M_D2XDT2_MAX ? -1.0e30 : 1.0e30);
+        break;
+    }
+    case M_D2XDT2_EXTREMA: {
+        real sq = sample_every * sample_every;
+        out[0] = buf[2] / sq; out[1] = buf[3] / sq;
+        buf[2] = real(-1.0e30); buf[3] = real(1.0e30);
+        break;
+    }
+    case M_SPIKE_COUNT:
+        out[0] = buf[1];
+        buf[1] = real(0);
+        break;
+    default: cubie_plugin_metric_save(id, param, buf, out, spp, sample_every); break;
+    }
+}
+
+template <typename Cfg>
+struct Summaries {
+    using real = typename Cfg::real;
+    static constexpr int H = Cfg::summ_buf_height;
+    static constexpr int NS = Cfg::n_summ_states;
+    static constexpr int NO = Cfg::n_summ_obs;
+    Vec<real, (NS * H > 0 ? NS * H : 1)> sbuf;
+    Vec<real, (NO * H > 0 ? NO * H : 1)> obuf;
+
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int i = 0; i < (NS * H > 0 ? NS * H : 1); ++i) sbuf[i] = real(0);
+#pragma unroll
+        for (int i = 0; i < (NO * H > 0 ? NO * H : 1); ++i) obuf[i] = real(0);
+    }
+
+    __device__ __forceinline__ static void chain_update(real value, real* buf, int idx) {
+        int off = 0;
+#pragma unroll
+        for (int m = 0; m < Cfg::n_metrics; ++m) {
+            metric_update<real>(Cfg::metric_ids[m], Cfg::metric_params[m], value, buf + off, idx);
+            off += Cfg::metric_buf_sizes[m];
+        }
+    }
+
+    __device__ __forceinline__ static void chain_save(real* buf, real* out) {
+        int off = 0, ooff = 0;
+#pragma unroll
+        for (int m = 0; m < Cfg::n_metrics; ++m) {
+            metric_save<real>(Cfg::metric_ids[m], Cfg::metric_params[m], buf + off, out + ooff,
+                              Cfg::samples_per_summary, real(Cfg::sample_summaries_every));
+            off += Cfg::metric_buf_sizes[m];
+            ooff += Cfg::metric_out_sizes[m];
+        }
+    }
+
+    template <typename S, typename O>
+    __device__ __forceinline__ void update(const S& state, const O& obs, int idx) {
+        if constexpr (Cfg::summarise_state) {
+#pragma unroll
+            for (int v = 0; v < NS; ++v) chain_update(state[Cfg::summ_state_idx[v]], &sbuf[v * H], idx);
+        }
+        if constexpr (Cfg::summarise_obs) {
+#pragma unroll
+            for (int v = 0; v < NO; ++v) chain_update(obs[Cfg::summ_obs_idx[v]], &obuf[v * H], idx);
+        }
+    }
+
+    // Flush every window buffer to its output row (or only reset, when
+    // `row` is past the allocation, as the reference's initial call does).
+    __device__ __forceinline__ void save(const cubie_solve_args& a, long long run, long long row) {
+        constexpr int OH = Cfg::summ_out_height;
+        const long long N = a.n_runs;
+        const bool store = row < a.summary_rows;
+        if constexpr (Cfg::summarise_state) {
+            real* out = static_cast<real*>(a.state_summ_out);
+#pragma unroll
+            for (int v = 0; v < NS; ++v) {
+                real tmp[OH > 0 ? OH : 1];
+                chain_save(&sbuf[v * H], tmp);
+                if (store) {
+#pragma unroll
+                    for (int k = 0; k < OH; ++k)
+                        out[(row * (NS * OH) + v * OH + k) * N + run] = tmp[k];
+                }
+            }
+        }
+        if constexpr (Cfg::summarise_obs) {
+            real* out = static_cast<real*>(a.obs_summ_out);
+#pragma unroll
+            for (int v = 0; v < NO; ++v) {
+                real tmp[OH > 0 ? OH : 1];
+                chain_save(&obuf[v * H], tmp);
+                if (store) {
+#pragma unroll
+                    for (int k = 0; k < OH; ++k)
+                        out[(row * (NO * OH) + v * OH + k) * N + run] = tmp[k];
+                }
+            }
+        }
+    }
+};
+
+// save_state_func: state columns, time column, observables, counters.
+template <typename Cfg, typename S, typename O>
+__device__ __forceinline__ void save_state(const cubie_solve_args& a, long long run, long long row,
+                                           const S& state, const O& obs, const int* counters,
+                                           typename Cfg::real t_prec) {
+    using real = typename Cfg::real;
+    if (row >= a.out_rows) return;
+    const long long N = a.n_runs;
+    constexpr int W = Cfg::n_saved_states + (Cfg::save_time ? 1 : 0);
+    if constexpr (Cfg::save_state) {
+        real* out = static_cast<real*>(a.state_out);
+#pragma unroll
+        for (int k = 0; k < Cfg::n_saved_states; ++k)
+            out[(row * W + k) * N + run] = state[Cfg::saved_state_idx[k]];
+    }
+    if constexpr (Cfg::save_time) {
+        real* out = static_cast<real*>(a.state_out);
+        out[(row * W + Cfg::n_saved_states) * N + run] = t_prec;
+    }
+    if constexpr (Cfg::save_obs) {
+        real* out = static_cast<real*>(a.observables_out);
+#pragma unroll
+        for (int k = 0; k < Cfg::n_saved_obs; ++k)
+            out[(row * Cfg::n_saved_obs + k) * N + run] = obs[Cfg::saved_obs_idx[k]];
+    }
+    if constexpr (Cfg::save_counters) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a.counters_out[(row * 4 + k) * N + run] = counters[k];
+    }
+}
+
+}  // namespace cubie

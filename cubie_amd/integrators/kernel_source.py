@@ -1,0 +1,267 @@
+"""Assemble the hiprtc translation unit for one solver configuration.
+
+The reference builds its kernel by closure capture: every setting becomes a
+Python constant baked into a Numba JIT (CUDAFactory.py:365-700,
+SingleIntegratorRunCore.build :835-882). Here the same settings become the
+members of one ``struct Cfg`` of compile-time constants, followed by the
+generated ``struct System`` and the hand-written device headers from
+``csrc/device``; hiprtc then folds every tableau coefficient, output index
+and controller constant into the gfx950 code.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .._utils import c_array, c_double, c_int_array, c_literal
+from ..odesystems.codegen import SystemCodegen
+from ..odesystems.symbolic import SymbolicODE
+from ..outputhandling.output_config import OutputConfig
+from ..outputhandling.summary_metrics import summary_metrics
+from .algorithms import ResolvedAlgorithm
+from .controllers import ControllerSettings
+from .timing import LoopTiming
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(_HERE), "csrc")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include")
+
+DEVICE_HEADERS = [
+    "common.hpp", "drivers.hpp", "controllers.hpp", "outputs.hpp", "linear.hpp",
+    "erk.hpp", "dirk.hpp", "firk.hpp", "rosenbrock.hpp", "loop.hpp",
+]
+
+FAMILY_STEPPER = {
+    "erk": "ERKStep", "dirk": "DIRKStep", "firk": "FIRKStep", "rosenbrock": "RosenbrockStep",
+}
+
+
+@dataclass
+class ImplicitSettings:
+    """Newton-Krylov / linear-solver settings (reference defaults)."""
+
+    newton_atol: float = 1e-7
+    newton_rtol: float = 1e-7
+    newton_max_iters: int = 100
+    krylov_atol: float = 1e-6
+    krylov_rtol: float = 1e-6
+    krylov_max_iters: int = 100
+    krylov_residual_reduction: Optional[float] = None
+    krylov_residual_floor: Optional[float] = None
+    linear_correction_type: str = "minimal_residual"
+    preconditioner_order: int = 2
+
+
+@dataclass
+class KernelSpec:
+    system: SymbolicODE
+    precision: type
+    algorithm: ResolvedAlgorithm
+    controller: ControllerSettings
+    timing: LoopTiming
+    outputs: OutputConfig
+    drivers: Optional[object] = None          # ArrayInterpolator
+    implicit: ImplicitSettings = field(default_factory=ImplicitSettings)
+    compensated: bool = False
+    per_run_rtol: bool = False
+    fp_contract: bool = False
+    block_size: int = 256
+
+    # -- helpers ---------------------------------------------------------------
+    def _lit(self, v) -> str:
+        return c_literal(v, self.precision)
+
+    def _real_name(self) -> str:
+        return "double" if self.precision is np.float64 else "float"
+
+    def config_struct(self) -> str:
+        p = self.precision
+        s = self.system
+        alg = self.algorithm
+        tab = alg.tableau
+        st = tab.stage_count
+        n = s.num_states
+        ctrl = self.controller
+        cc = ctrl.constants()
+        out = self.outputs
+        tm = self.timing
+        L: List[str] = []
+        a = L.append
+        a("struct Cfg {")
+        a(f"  using real = {self._real_name()};")
+        norm_real = "double" if (self.compensated or p is np.float64) else "float"
+        a(f"  using norm_real = {norm_real};")
+        a(f"  static constexpr int n = {n};")
+        a(f"  static constexpr int n_params = {s.num_parameters};")
+        a(f"  static constexpr int n_obs = {s.num_observables};")
+        a(f"  static constexpr int n_drivers = {s.num_drivers};")
+        adaptive = ctrl.is_adaptive
+        n_error = n if alg.is_adaptive else 0
+        a(f"  static constexpr int n_error = {n_error};")
+        a(f"  static constexpr bool adaptive = {str(adaptive).lower()};")
+        a(f"  static constexpr bool compensated = {str(self.compensated).lower()};")
+        a(f"  static constexpr bool per_run_rtol = {str(self.per_run_rtol).lower()};")
+        # controller
+        a(f"  static constexpr int controller = {ctrl.id};")
+        a(f"  static constexpr real atol[{max(n, 1)}] = {c_array(ctrl.atol, p)};")
+        a(f"  static constexpr real rtol[{max(n, 1)}] = {c_array(ctrl.rtol, p)};")
+        for key in ("inv_n", "kp_exp", "ki_exp", "kd_exp", "order_exponent", "safety",
+                    "min_gain", "max_gain", "dt_min", "dt_max", "deadband_min",
+                    "deadband_max", "gain_numerator"):
+            a(f"  static constexpr real {key} = {self._lit(cc[key])};")
+        a(f"  static constexpr bool deadband_disabled = {str(cc['deadband_disabled']).lower()};")
+        a(f"  static constexpr int newton_target_iters = {int(ctrl.newton_target_iters)};")
+        a(f"  static constexpr real dt0 = {self._lit(ctrl.dt)};")
+        # loop timing
+        se = tm.save_every if tm.save_every is not None else 0.0
+        ss = tm.sample_summaries_every if tm.sample_summaries_every is not None else 0.0
+        a(f"  static constexpr real save_every = {self._lit(se)};")
+        a(f"  static constexpr real sample_summaries_every = {self._lit(ss)};")
+        a(f"  static constexpr int samples_per_summary = {max(tm.samples_per_summary, 1) if out.save_summaries else 1};")
+        a(f"  static constexpr bool save_last = {str(tm.save_last).lower()};")
+        a(f"  static constexpr bool save_regularly = {str(tm.save_regularly).lower()};")
+        a(f"  static constexpr bool summarise_regularly = {str(tm.summarise_regularly).lower()};")
+        # outputs
+        a(f"  static constexpr bool save_state = {str(out.save_state).lower()};")
+        a(f"  static constexpr bool save_obs = {str(out.save_observables).lower()};")
+        a(f"  static constexpr bool save_time = {str(out.save_time).lower()};")
+        a(f"  static constexpr bool save_counters = {str(out.save_counters).lower()};")
+        a(f"  static constexpr bool summarise_state = {str(out.summarise_state).lower()};")
+        a(f"  static constexpr bool summarise_obs = {str(out.summarise_observables).lower()};")
+        ssi = out.saved_state_indices if out.save_state else []
+        soi = out.saved_observable_indices if out.save_observables else []
+        smi = out.summarised_state_indices if out.summarise_state else []
+        soi2 = out.summarised_observable_indices if out.summarise_observables else []
+        a(f"  static constexpr int n_saved_states = {len(ssi)};")
+        a(f"  static constexpr int saved_state_idx[{max(len(ssi), 1)}] = {c_int_array(ssi)};")
+        a(f"  static constexpr int n_saved_obs = {len(soi)};")
+        a(f"  static constexpr int saved_obs_idx[{max(len(soi), 1)}] = {c_int_array(soi)};")
+        a(f"  static constexpr int n_summ_states = {len(smi)};")
+        a(f"  static constexpr int summ_state_idx[{max(len(smi), 1)}] = {c_int_array(smi)};")
+        a(f"  static constexpr int n_summ_obs = {len(soi2)};")
+        a(f"  static constexpr int summ_obs_idx[{max(len(soi2), 1)}] = {c_int_array(soi2)};")
+        mets = out.metrics
+        ids = [summary_metrics[m].metric_id for m, _ in mets]
+        prm = [pp for _, pp in mets]
+        bsz = [summary_metrics[m].buffer_len(pp) for m, pp in mets]
+        osz = [summary_metrics[m].output_len(pp) for m, pp in mets]
+        a(f"  static constexpr int n_metrics = {len(mets)};")
+        a(f"  static constexpr int metric_ids[{max(len(mets), 1)}] = {c_int_array(ids)};")
+        a(f"  static constexpr int metric_params[{max(len(mets), 1)}] = {c_int_array(prm)};")
+        a(f"  static constexpr int metric_buf_sizes[{max(len(mets), 1)}] = {c_int_array(bsz)};")
+        a(f"  static constexpr int metric_out_sizes[{max(len(mets), 1)}] = {c_int_array(osz)};")
+        a(f"  static constexpr int summ_buf_height = {sum(bsz)};")
+        a(f"  static constexpr int summ_out_height = {sum(osz)};")
+        # drivers
+        d = self.drivers
+        if d is not None and s.num_drivers > 0:
+            dc = d.device_constants()
+            a(f"  static constexpr int drv_order = {d.order};")
+            a(f"  static constexpr int drv_segments = {d.num_segments};")
+            a(f"  static constexpr bool drv_wrap = {str(d.wrap).lower()};")
+            a(f"  static constexpr real drv_inv_resolution = {self._lit(dc['inv_resolution'])};")
+            a(f"  static constexpr real drv_evaluation_start = {self._lit(dc['evaluation_start'])};")
+        else:
+            a("  static constexpr int drv_order = 0;")
+            a("  static constexpr int drv_segments = 1;")
+            a("  static constexpr bool drv_wrap = true;")
+            a("  static constexpr real drv_inv_resolution = real(1);")
+            a("  static constexpr real drv_evaluation_start = real(0);")
+        # algorithm
+        a(f"  static constexpr int stages = {st};")
+        a(f"  static constexpr int order = {tab.order};")
+        a(f"  static constexpr bool has_error = {str(alg.is_adaptive).lower()};")
+        a(f"  static constexpr bool fsal = {str(tab.first_same_as_last).lower()};")
+        br = tab.b_matches_a_row
+        bhr = tab.b_hat_matches_a_row if alg.is_adaptive else None
+        a(f"  static constexpr int b_row = {-1 if br is None else br};")
+        a(f"  static constexpr int b_hat_row = {-1 if bhr is None else bhr};")
+        a(f"  static constexpr bool accumulates_output = {str(br is None).lower()};")
+        a(f"  static constexpr bool accumulates_error = {str(bhr is None).lower()};")
+        rows = ", ".join(c_array(r, p) for r in tab.a)
+        a(f"  static constexpr real A[{st}][{st}] = {{{rows}}};")
+        a(f"  static constexpr real b[{st}] = {c_array(tab.b, p)};")
+        a(f"  static constexpr real c[{st}] = {c_array(tab.c, p)};")
+        dvec = tab.d if alg.is_adaptive else [0.0] * st
+        a(f"  static constexpr real d[{st}] = {c_array(dvec, p)};")
+        bh = tab.b_hat if tab.b_hat is not None else [0.0] * st
+        a(f"  static constexpr real b_hat[{st}] = {c_array(bh, p)};")
+        if alg.family == "rosenbrock":
+            crow = ", ".join(c_array(r, p) for r in tab.C)
+            a(f"  static constexpr real rbC[{st}][{st}] = {{{crow}}};")
+            a(f"  static constexpr real gamma = {self._lit(tab.gamma)};")
+            a(f"  static constexpr real gamma_stages[{st}] = {c_array(tab.gamma_stages, p)};")
+        # implicit solver settings
+        im = self.implicit
+        lin = {"minimal_residual": 0, "steepest_descent": 1, "bicgstab": 2}[im.linear_correction_type]
+        a(f"  static constexpr int newton_max_iters = {int(im.newton_max_iters)};")
+        a(f"  static constexpr int krylov_max_iters = {int(im.krylov_max_iters)};")
+        a(f"  static constexpr int linear_solver = {lin};")
+        a(f"  static constexpr int preconditioner_order = {int(im.preconditioner_order)};")
+        a(f"  static constexpr real newton_atol = {self._lit(im.newton_atol)};")
+        a(f"  static constexpr real newton_rtol = {self._lit(im.newton_rtol)};")
+        a(f"  static constexpr real krylov_atol = {self._lit(im.krylov_atol)};")
+        a(f"  static constexpr real krylov_rtol = {self._lit(im.krylov_rtol)};")
+        red = im.krylov_residual_reduction if im.krylov_residual_reduction is not None else float(np.finfo(p).eps)
+        flo = im.krylov_residual_floor if im.krylov_residual_floor is not None else float(np.sqrt(np.finfo(p).eps))
+        a(f"  static constexpr real krylov_reduction = {self._lit(red)};")
+        a(f"  static constexpr real krylov_floor = {self._lit(flo)};")
+        a("};")
+        return "\n".join(L) + "\n"
+
+    def plugin_hooks(self) -> str:
+        """Device hooks for plugin summary metrics (empty when none)."""
+        upd, sav = [], []
+        for name, _ in self.outputs.metrics:
+            m = summary_metrics[name]
+            if m.update_body is None:
+                continue
+            upd.append(f"  if (id == {m.metric_id}) {{ {m.update_body} }}")
+            sav.append(f"  if (id == {m.metric_id}) {{ {m.save_body} }}")
+        u = "\n".join(upd)
+        s = "\n".join(sav)
+        return (
+            "template <typename real>\n"
+            "__device__ __forceinline__ void cubie_plugin_metric_update(int id, int param, real value, real* buf, int idx) {\n"
+            f"{u}\n  (void)id; (void)param; (void)value; (void)buf; (void)idx;\n}}\n"
+            "template <typename real>\n"
+            "__device__ __forceinline__ void cubie_plugin_metric_save(int id, int param, real* buf, real* out, int spp, real sample_every) {\n"
+            f"{s}\n  (void)id; (void)param; (void)buf; (void)out; (void)spp; (void)sample_every;\n}}\n"
+        )
+
+    def source(self) -> str:
+        parts = ["// Generated translation unit: configuration + system + device library.",
+                 "#define CUBIE_DEVICE_COMPILE 1"]
+        with open(os.path.join(INCLUDE, "cubie_hip.h")) as f:
+            parts.append(f.read())
+        parts.append(self.config_struct())
+        parts.append("using real = Cfg::real;")
+        gen = SystemCodegen(self.system, self.precision)
+        parts.append(gen.emit())
+        self.op_counts = gen.op_counts
+        parts.append(self.plugin_hooks())
+        for h in DEVICE_HEADERS:
+            with open(os.path.join(CSRC, "device", h)) as f:
+                parts.append(f"// ---- {h} ----")
+                parts.append(f.read())
+        stepper = FAMILY_STEPPER[self.algorithm.family]
+        bs = int(self.block_size)
+        parts.append(
+            f"using StepperT = cubie::{stepper}<Cfg, System>;\n"
+            f"extern \"C\" __global__ void __launch_bounds__({bs}) cubie_integrate(cubie_solve_args a) {{\n"
+            "  cubie::integrate_plain<Cfg, System, StepperT>(a);\n}\n"
+            f"extern \"C\" __global__ void __launch_bounds__({bs}) cubie_integrate_persistent(cubie_solve_args a) {{\n"
+            "  cubie::integrate_persistent<Cfg, System, StepperT>(a);\n}\n")
+        return "\n".join(parts)
+
+    def compile_options(self) -> List[str]:
+        opts = ["--offload-arch=gfx950", "-O3", "-std=c++17",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                "-Wno-unused-parameter", "-Wno-unused-variable"]
+        opts.append("-ffp-contract=fast" if self.fp_contract else "-ffp-contract=off")
+        return opts

@@ -1,0 +1,312 @@
+"""HIP C++ emission for a :class:`SymbolicODE`.
+
+The reference generates Numba-Python device functions from its own IR
+(odesystems/symbolic/codegen/dxdt.py DXDT_TEMPLATE, linear_operators.py,
+time_derivative.py; engine/printer.py). Here the same system functions are
+emitted as HIP C++ for hiprtc:
+
+* ``dxdt(y, p, d, obs, dy, t)``   f(t, y) (also refreshes the observables)
+* ``observables(y, p, d, obs, t)``
+* ``jvp(y, p, d, v, jv, t)``       J(y) v from the symbolic Jacobian,
+                                   contracted with v (no dense J in memory)
+* ``dfdt(y, p, d, dd, out, t)``    df/dt with driver derivatives dd
+* ``jac(y, p, d, J, t)``           dense J (row-major), for the adjoint
+* ``jacp(y, p, d, Jp, t)``         dense df/dparams (row-major)
+
+Every array argument is a pointer into a register-resident vector of the
+calling loop; after inlining the indices are constants. Constants of the
+system become exact literals. Common subexpressions are shared with
+``sympy.cse`` and counted by :func:`count_ops` for the roofline flop model.
+"""
+
+from __future__ import annotations
+
+import re
+from typing import Dict, Iterable, List, Sequence, Tuple
+
+import numpy as np
+import sympy as sp
+from sympy.printing.c import C99CodePrinter
+
+from .._utils import c_literal
+from .symbolic import TIME_SYMBOL, SymbolicODE
+
+_FUNCS = {
+    "exp": "exp", "log": "log", "sqrt": "sqrt", "sin": "sin", "cos": "cos",
+    "tan": "tan", "asin": "asin", "acos": "acos", "atan": "atan", "atan2": "atan2",
+    "sinh": "sinh", "cosh": "cosh", "tanh": "tanh", "floor": "floor",
+    "ceiling": "ceil", "Abs": "fabs",
+}
+
+
+class HipPrinter(C99CodePrinter):
+    """C99 printer with exact typed literals and integer-power expansion."""
+
+    def __init__(self, precision, user_functions=None):
+        super().__init__()
+        self.precision = precision
+        self.user_templates = dict(user_functions or {})
+
+    def _lit(self, v: float) -> str:
+        return c_literal(v, self.precision)
+
+    def _print_Float(self, expr):
+        return self._lit(float(expr))
+
+    def _print_Integer(self, expr):
+        return f"real({int(expr)})"
+
+    def _print_Rational(self, expr):
+        return self._lit(float(expr.p) / float(expr.q))
+
+    def _print_Half(self, expr):
+        return self._lit(0.5)
+
+    def _print_NegativeOne(self, expr):
+        return "real(-1)"
+
+    def _print_One(self, expr):
+        return "real(1)"
+
+    def _print_Zero(self, expr):
+        return "real(0)"
+
+    def _print_Pi(self, expr):
+        return self._lit(float(sp.pi))
+
+    def _print_Exp1(self, expr):
+        return self._lit(float(sp.E))
+
+    def _print_Pow(self, expr):
+        base, e = expr.as_base_exp()
+        b = self.parenthesize(base, 100)
+        if e.is_Integer:
+            k = int(e)
+            if k == 0:
+                return "real(1)"
+            if 0 < k <= 8:
+                return "(" + "*".join([b] * k) + ")"
+            if -8 <= k < 0:
+                return "(real(1)/(" + "*".join([b] * (-k)) + "))"
+            return f"pow({self._print(base)}, real({k}))"
+        if e == sp.Rational(1, 2):
+            return f"sqrt({self._print(base)})"
+        if e == sp.Rational(-1, 2):
+            return f"(real(1)/sqrt({self._print(base)}))"
+        return f"pow({self._print(base)}, {self._print(e)})"
+
+    def _print_Function(self, expr):
+        name = expr.func.__name__
+        if name in self.user_templates:
+            args = [self._print(a) for a in expr.args]
+            return "(" + self.user_templates[name].format(*args) + ")"
+        if name in _FUNCS:
+            args = ", ".join(self._print(a) for a in expr.args)
+            return f"{_FUNCS[name]}({args})"
+        return super()._print_Function(expr)
+
+    def _print_Min(self, expr):
+        args = [self._print(a) for a in expr.args]
+        out = args[0]
+        for a in args[1:]:
+            out = f"fmin({out}, {a})"
+        return out
+
+    def _print_Max(self, expr):
+        args = [self._print(a) for a in expr.args]
+        out = args[0]
+        for a in args[1:]:
+            out = f"fmax({out}, {a})"
+        return out
+
+    def _print_sign(self, expr):
+        x = self._print(expr.args[0])
+        return f"(real(({x}) > real(0)) - real(({x}) < real(0)))"
+
+    def _print_Heaviside(self, expr):
+        x = self._print(expr.args[0])
+        return f"(({x}) > real(0) ? real(1) : (({x}) < real(0) ? real(0) : {self._lit(0.5)}))"
+
+    def _print_Piecewise(self, expr):
+        parts = []
+        for e, c in expr.args:
+            parts.append((self._print(e), self._print(c)))
+        out = parts[-1][0]
+        for e, c in reversed(parts[:-1]):
+            out = f"(({c}) ? ({e}) : ({out}))"
+        return out
+
+    def _print_Relational(self, expr):
+        return f"({self._print(expr.lhs)} {expr.rel_op} {self._print(expr.rhs)})"
+
+    def _print_BooleanTrue(self, expr):
+        return "true"
+
+    def _print_BooleanFalse(self, expr):
+        return "false"
+
+    def _print_Symbol(self, expr):
+        return _cname(str(expr))
+
+
+def _cname(name: str) -> str:
+    return "v_" + re.sub(r"[^A-Za-z0-9_]", "_", name)
+
+
+def count_ops(exprs: Sequence[sp.Expr]) -> Dict[str, int]:
+    """Algorithmic operation counts of a list of (CSE'd) expressions.
+
+    add/sub/mul/div = 1 flop each, an integer power k costs k-1 multiplies,
+    transcendental calls are tallied separately (one each).
+    """
+    counts = {"flops": 0, "transcendental": 0}
+
+    def walk(e):
+        if e.is_Atom:
+            return
+        if e.is_Add:
+            counts["flops"] += len(e.args) - 1
+        elif e.is_Mul:
+            counts["flops"] += len(e.args) - 1
+        elif e.is_Pow:
+            base, ex = e.as_base_exp()
+            if ex.is_Integer and abs(int(ex)) <= 8:
+                counts["flops"] += max(abs(int(ex)) - 1, 0) + (1 if int(ex) < 0 else 0)
+            elif ex == sp.Rational(1, 2):
+                counts["flops"] += 1
+            else:
+                counts["transcendental"] += 1
+        elif isinstance(e, sp.Function):
+            counts["transcendental"] += 1
+        for a in e.args:
+            walk(a)
+
+    for e in exprs:
+        walk(e)
+    return counts
+
+
+class SystemCodegen:
+    """Generate the ``System`` struct of device functions for one system."""
+
+    def __init__(self, system: SymbolicODE, precision=None):
+        self.system = system
+        self.precision = precision or system.precision
+        self.printer = HipPrinter(self.precision, system.user_functions)
+        self.op_counts: Dict[str, Dict[str, int]] = {}
+
+    # -- helpers -------------------------------------------------------------
+    def _bindings(self, use_drivers=True) -> List[str]:
+        s = self.system
+        lines = []
+        for i, nm in enumerate(s.state_names):
+            lines.append(f"    [[maybe_unused]] const real {_cname(nm)} = y[{i}];")
+        for i, nm in enumerate(s.parameter_names):
+            lines.append(f"    [[maybe_unused]] const real {_cname(nm)} = p[{i}];")
+        for nm in s.constant_names:
+            lines.append(f"    [[maybe_unused]] constexpr real {_cname(nm)} = {c_literal(s.constants[nm], self.precision)};")
+        if use_drivers:
+            for i, nm in enumerate(s.driver_names):
+                lines.append(f"    [[maybe_unused]] const real {_cname(nm)} = d[{i}];")
+        lines.append("    [[maybe_unused]] const real v_t = t;")
+        return lines
+
+    def _emit_block(self, targets: List[Tuple[str, sp.Expr]], key: str) -> List[str]:
+        exprs = [e for _, e in targets]
+        if not exprs:
+            self.op_counts[key] = {"flops": 0, "transcendental": 0}
+            return []
+        syms = sp.numbered_symbols("cse_")
+        repl, reduced = sp.cse(exprs, symbols=syms, optimizations="basic")
+        self.op_counts[key] = count_ops([r for _, r in repl] + list(reduced))
+        lines = []
+        for sym, e in repl:
+            lines.append(f"    const real {_cname(str(sym))} = {self.printer.doprint(e)};")
+        for (target, _), e in zip(targets, reduced):
+            lines.append(f"    {target} = {self.printer.doprint(e)};")
+        return lines
+
+    def _full_env(self) -> Dict[sp.Symbol, sp.Expr]:
+        env: Dict[sp.Symbol, sp.Expr] = {}
+        for a in self.system.assignments:
+            env[a.lhs] = a.rhs.xreplace(env)
+        return env
+
+    def _unused(self, body: List[str]) -> List[str]:
+        return body
+
+    # -- functions -------------------------------------------------------------
+    def emit(self) -> str:
+        s = self.system
+        env = self._full_env()
+        dx = [env[d] for d in s.derivative_symbols]
+        obs = [env[o] for o in s.observable_symbols]
+        n = s.num_states
+        parts = ["struct System {"]
+        # dxdt (also writes observables so a stage leaves them current)
+        targets = [(f"dy[{i}]", e) for i, e in enumerate(dx)]
+        targets += [(f"obs[{i}]", e) for i, e in enumerate(obs)]
+        parts.append("  __device__ static __forceinline__ void dxdt(const real* __restrict__ y, "
+                     "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ obs, "
+                     "real* __restrict__ dy, real t) {")
+        parts += self._bindings()
+        parts += self._emit_block(targets, "dxdt")
+        parts.append("    (void)y; (void)p; (void)d; (void)obs; (void)t;")
+        parts.append("  }")
+        # observables
+        parts.append("  __device__ static __forceinline__ void observables(const real* __restrict__ y, "
+                     "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ obs, real t) {")
+        parts += self._bindings()
+        parts += self._emit_block([(f"obs[{i}]", e) for i, e in enumerate(obs)], "observables")
+        parts.append("    (void)y; (void)p; (void)d; (void)obs; (void)t;")
+        parts.append("  }")
+        # jvp: J v
+        vs = [sp.Symbol(f"jv_in_{i}", real=True) for i in range(n)]
+        jv = []
+        for fi in dx:
+            jv.append(sp.Add(*[sp.diff(fi, yj) * vj for yj, vj in zip(s.state_symbols, vs)]))
+        parts.append("  __device__ static __forceinline__ void jvp(const real* __restrict__ y, "
+                     "const real* __restrict__ p, const real* __restrict__ d, const real* __restrict__ v, "
+                     "real* __restrict__ jv, real t) {")
+        parts += self._bindings()
+        for i in range(n):
+            parts.append(f"    [[maybe_unused]] const real {_cname(str(vs[i]))} = v[{i}];")
+        parts += self._emit_block([(f"jv[{i}]", e) for i, e in enumerate(jv)], "jvp")
+        parts.append("    (void)y; (void)p; (void)d; (void)v; (void)t;")
+        parts.append("  }")
+        # dfdt
+        dds = [sp.Symbol(f"dd_in_{i}", real=True) for i in range(s.num_drivers)]
+        dfdt = []
+        for fi in dx:
+            e = sp.diff(fi, TIME_SYMBOL)
+            for dsym, ddsym in zip(s.driver_symbols, dds):
+                e += sp.diff(fi, dsym) * ddsym
+            dfdt.append(e)
+        parts.append("  __device__ static __forceinline__ void dfdt(const real* __restrict__ y, "
+                     "const real* __restrict__ p, const real* __restrict__ d, const real* __restrict__ dd, "
+                     "real* __restrict__ out, real t) {")
+        parts += self._bindings()
+        for i in range(s.num_drivers):
+            parts.append(f"    [[maybe_unused]] const real {_cname(str(dds[i]))} = dd[{i}];")
+        parts += self._emit_block([(f"out[{i}]", e) for i, e in enumerate(dfdt)], "dfdt")
+        parts.append("    (void)y; (void)p; (void)d; (void)dd; (void)t;")
+        parts.append("  }")
+        # dense Jacobians for the adjoint
+        J = [(f"J[{i * n + j}]", sp.diff(dx[i], s.state_symbols[j])) for i in range(n) for j in range(n)]
+        parts.append("  __device__ static __forceinline__ void jac(const real* __restrict__ y, "
+                     "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ J, real t) {")
+        parts += self._bindings()
+        parts += self._emit_block(J, "jac")
+        parts.append("    (void)y; (void)p; (void)d; (void)J; (void)t;")
+        parts.append("  }")
+        npar = s.num_parameters
+        Jp = [(f"Jp[{i * max(npar, 1) + j}]", sp.diff(dx[i], s.parameter_symbols[j]))
+              for i in range(n) for j in range(npar)]
+        parts.append("  __device__ static __forceinline__ void jacp(const real* __restrict__ y, "
+                     "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ Jp, real t) {")
+        parts += self._bindings()
+        parts += self._emit_block(Jp, "jacp")
+        parts.append("    (void)y; (void)p; (void)d; (void)Jp; (void)t;")
+        parts.append("  }")
+        parts.append("};")
+        return "\n".join(parts) + "\n"

@@ -1,0 +1,421 @@
+"""String / SymPy front end: equations -> a symbolic system description.
+
+Mirrors the reference's ``create_ODE_system`` / ``SymbolicODE`` entry points
+(src/cubie/odesystems/symbolic/symbolicODE.py:102-200) for the explicit-ODE
+case: equations are given as ``"lhs = rhs"`` strings (one string with
+newlines, or an iterable of strings) or as a Python callable
+``f(t, y, ...)``; ``d<state>`` on the left-hand side marks a derivative,
+any other left-hand side is an auxiliary, and an auxiliary listed in
+``observables`` is exported. States, parameters, constants and drivers may be
+declared as name lists or as ``name -> default`` mappings. With
+``strict=False`` undeclared right-hand-side symbols become parameters and
+undeclared ``d<name>`` targets become states, as in the reference's
+non-strict parser (parsing/parser.py:627-760).
+
+The device code generated from this description lives in
+:mod:`cubie_amd.odesystems.codegen`.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import inspect
+import re
+from dataclasses import dataclass, field
+from typing import Callable, Dict, Iterable, List, Mapping, Optional, Sequence, Tuple, Union
+
+import numpy as np
+import sympy as sp
+
+from .._utils import precision_from
+
+TIME_SYMBOL = sp.Symbol("t", real=True)
+
+_SYMPY_FUNCTIONS = {
+    "exp": sp.exp, "log": sp.log, "ln": sp.log, "sqrt": sp.sqrt,
+    "sin": sp.sin, "cos": sp.cos, "tan": sp.tan,
+    "asin": sp.asin, "acos": sp.acos, "atan": sp.atan, "atan2": sp.atan2,
+    "sinh": sp.sinh, "cosh": sp.cosh, "tanh": sp.tanh,
+    "abs": sp.Abs, "Abs": sp.Abs, "fabs": sp.Abs,
+    "min": sp.Min, "Min": sp.Min, "max": sp.Max, "Max": sp.Max,
+    "floor": sp.floor, "ceil": sp.ceiling, "ceiling": sp.ceiling,
+    "sign": sp.sign, "Piecewise": sp.Piecewise, "pi": sp.pi, "E": sp.E,
+    "heaviside": sp.Heaviside, "Heaviside": sp.Heaviside,
+}
+
+DRIVER_SETTING_KEYS = {"time", "dt", "wrap", "order", "boundary_condition", "t0",
+                       "driver_sample_period"}
+
+
+@dataclass
+class SystemSizes:
+    """Counts of each symbol category (reference: odesystems/ODEData.py)."""
+
+    states: int
+    parameters: int
+    constants: int
+    observables: int
+    drivers: int
+
+
+@dataclass
+class Assignment:
+    lhs: sp.Symbol
+    rhs: sp.Expr
+    kind: str  # "derivative", "observable", "auxiliary"
+
+
+def _as_default_map(values, default=0.0) -> Dict[str, float]:
+    if values is None:
+        return {}
+    if isinstance(values, Mapping):
+        out = {}
+        for k, v in values.items():
+            arr = np.asarray(v, dtype=np.float64)
+            out[str(k)] = float(arr.flat[0]) if arr.size else float(default)
+        return out
+    return {str(k): float(default) for k in values}
+
+
+def _split_equations(dxdt: Union[str, Iterable[str]]) -> List[str]:
+    if isinstance(dxdt, str):
+        raw = re.split(r"[\n;]", dxdt)
+    else:
+        raw = []
+        for item in dxdt:
+            raw.extend(re.split(r"[\n;]", str(item)))
+    lines = []
+    for line in raw:
+        line = line.split("#", 1)[0].strip()
+        if line:
+            lines.append(line)
+    return lines
+
+
+class SymbolicODE:
+    """Symbolic ODE system: ordered symbols, assignments and defaults.
+
+    The attribute names follow the reference's system objects so that the
+    solver layer can read sizes, labels and default values the same way
+    (``num_states``, ``sizes``, ``initial_values``, ``parameters``, ...).
+    """
+
+    def __init__(self, *, states: Dict[str, float], parameters: Dict[str, float],
+                 constants: Dict[str, float], drivers: List[str],
+                 observables: List[str], assignments: List[Assignment],
+                 precision=np.float32, name: Optional[str] = None,
+                 driver_defaults: Optional[dict] = None,
+                 user_functions: Optional[Dict[str, str]] = None):
+        self.precision = precision_from(precision)
+        self.state_names: List[str] = list(states)
+        self.parameter_names: List[str] = list(parameters)
+        self.constant_names: List[str] = list(constants)
+        self.driver_names: List[str] = list(drivers)
+        self.observable_names: List[str] = list(observables)
+        self._state_defaults = dict(states)
+        self._parameter_defaults = dict(parameters)
+        self._constants = dict(constants)
+        self.assignments = assignments
+        self.driver_defaults = driver_defaults
+        self.user_functions = dict(user_functions or {})
+        self.state_symbols = [sp.Symbol(s, real=True) for s in self.state_names]
+        self.parameter_symbols = [sp.Symbol(s, real=True) for s in self.parameter_names]
+        self.constant_symbols = [sp.Symbol(s, real=True) for s in self.constant_names]
+        self.driver_symbols = [sp.Symbol(s, real=True) for s in self.driver_names]
+        self.observable_symbols = [sp.Symbol(s, real=True) for s in self.observable_names]
+        self.derivative_symbols = [sp.Symbol("d" + s, real=True) for s in self.state_names]
+        self._validate()
+        self.name = name or ("system_" + self.fn_hash[:12])
+
+    # -- construction ----------------------------------------------------
+    @classmethod
+    def create(cls, dxdt, states=None, observables=None, parameters=None,
+               constants=None, drivers=None, user_functions=None,
+               name=None, precision=np.float32, strict=False, **_unused):
+        if callable(dxdt) and not isinstance(dxdt, str):
+            return cls._from_callable(dxdt, states=states, observables=observables,
+                                      parameters=parameters, constants=constants,
+                                      drivers=drivers, name=name, precision=precision,
+                                      strict=strict)
+        if states is None and strict:
+            raise ValueError(
+                "No state symbols were provided - if you want to build a model "
+                "from a set of equations alone, set strict=False")
+        state_map = _as_default_map(states)
+        param_map = _as_default_map(parameters)
+        const_map = _as_default_map(constants)
+        obs_list = [str(o) for o in (observables or [])]
+        driver_defaults = None
+        if drivers is None:
+            driver_list: List[str] = []
+        elif isinstance(drivers, Mapping):
+            driver_defaults = dict(drivers)
+            driver_list = [k for k in drivers if k not in DRIVER_SETTING_KEYS]
+            if not driver_list:
+                raise ValueError("Driver dictionary must include at least one driver symbol.")
+        else:
+            driver_list = [str(d) for d in drivers]
+
+        lines = _split_equations(dxdt)
+        parsed: List[Tuple[str, str]] = []
+        for line in lines:
+            if "=" not in line:
+                raise ValueError(f"Equation '{line}' is not in 'lhs = rhs' form")
+            lhs, rhs = line.split("=", 1)
+            lhs = lhs.strip()
+            if not re.fullmatch(r"[A-Za-z_][A-Za-z0-9_]*", lhs):
+                raise ValueError(f"Invalid left-hand side '{lhs}'")
+            parsed.append((lhs, rhs.strip()))
+
+        lhs_names = [l for l, _ in parsed]
+        # classify left-hand sides
+        derivs: Dict[str, str] = {}
+        for lhs in lhs_names:
+            if lhs.startswith("d") and lhs[1:] in state_map:
+                derivs[lhs] = lhs[1:]
+            elif (not strict and lhs.startswith("d") and len(lhs) > 1
+                  and lhs not in obs_list and lhs[1:] not in param_map
+                  and lhs[1:] not in const_map and lhs[1:] not in driver_list
+                  and lhs[1:] not in obs_list):
+                derivs[lhs] = lhs[1:]
+                state_map.setdefault(lhs[1:], 0.0)
+        missing = [s for s in state_map if ("d" + s) not in derivs]
+        if missing:
+            raise ValueError(f"No derivative equation given for state(s) {missing}")
+
+        known = set(state_map) | set(param_map) | set(const_map) | set(driver_list) | {"t"}
+        known |= set(lhs_names)
+        local = dict(_SYMPY_FUNCTIONS)
+        user_functions = dict(user_functions or {})
+        for fname in user_functions:
+            local[fname] = sp.Function(fname)
+        for nm in known:
+            local[nm] = sp.Symbol(nm, real=True)
+        assignments: List[Assignment] = []
+        inferred_params: List[str] = []
+        for lhs, rhs in parsed:
+            expr = _parse_rhs(rhs, local)
+            for sym in expr.free_symbols:
+                nm = str(sym)
+                if nm not in known:
+                    if strict:
+                        raise ValueError(f"Undeclared symbol '{nm}' in equation for '{lhs}'")
+                    inferred_params.append(nm)
+                    known.add(nm)
+                    local[nm] = sp.Symbol(nm, real=True)
+            if lhs in derivs:
+                kind = "derivative"
+            elif lhs in obs_list:
+                kind = "observable"
+            else:
+                kind = "auxiliary"
+            assignments.append(Assignment(sp.Symbol(lhs, real=True), expr, kind))
+        for nm in inferred_params:
+            param_map.setdefault(nm, 0.0)
+        for o in obs_list:
+            if o not in lhs_names:
+                raise ValueError(f"Observable '{o}' has no defining equation")
+        ordered = _topological(assignments)
+        return cls(states=state_map, parameters=param_map, constants=const_map,
+                   drivers=driver_list, observables=obs_list, assignments=ordered,
+                   precision=precision, name=name, driver_defaults=driver_defaults,
+                   user_functions=user_functions)
+
+    @classmethod
+    def _from_callable(cls, fn, *, states, observables, parameters, constants,
+                       drivers, name, precision, strict):
+        """``f(t, y, *args)`` returning a sequence or dict of derivatives."""
+        state_map = _as_default_map(states)
+        if not state_map:
+            raise ValueError("Callable input needs declared states")
+        param_map = _as_default_map(parameters)
+        const_map = _as_default_map(constants)
+        driver_list = [str(d) for d in (drivers or [])]
+        sig = inspect.signature(fn)
+        argnames = list(sig.parameters)[2:]
+        t = TIME_SYMBOL
+        ys = [sp.Symbol(s, real=True) for s in state_map]
+
+        class _Y(list):
+            def __getitem__(self, k):
+                if isinstance(k, str):
+                    return ys[list(state_map).index(k)]
+                return list.__getitem__(self, k)
+
+            def __getattr__(self, k):
+                return ys[list(state_map).index(k)]
+
+        extra = []
+        for a in argnames:
+            if a not in param_map and a not in const_map and a not in driver_list:
+                if strict:
+                    raise ValueError(f"Undeclared callable argument '{a}'")
+                param_map[a] = 0.0
+            extra.append(sp.Symbol(a, real=True))
+        out = fn(t, _Y(ys), *extra)
+        if isinstance(out, Mapping):
+            exprs = [sp.sympify(out[k]) for k in state_map]
+        else:
+            exprs = [sp.sympify(e) for e in out]
+        if len(exprs) != len(state_map):
+            raise ValueError("Callable must return one derivative per state")
+        assignments = [Assignment(sp.Symbol("d" + s, real=True), e, "derivative")
+                       for s, e in zip(state_map, exprs)]
+        return cls(states=state_map, parameters=param_map, constants=const_map,
+                   drivers=driver_list, observables=[], assignments=assignments,
+                   precision=precision, name=name)
+
+    def _validate(self):
+        names = (self.state_names + self.parameter_names + self.constant_names
+                 + self.driver_names)
+        dup = {n for n in names if names.count(n) > 1}
+        if dup:
+            raise ValueError(f"Symbols declared in more than one category: {sorted(dup)}")
+        if "t" in names:
+            raise ValueError("'t' is reserved for time")
+
+    # -- reference-style accessors -----------------------------------------
+    @property
+    def num_states(self) -> int:
+        return len(self.state_names)
+
+    @property
+    def num_parameters(self) -> int:
+        return len(self.parameter_names)
+
+    @property
+    def num_observables(self) -> int:
+        return len(self.observable_names)
+
+    @property
+    def num_drivers(self) -> int:
+        return len(self.driver_names)
+
+    @property
+    def sizes(self) -> SystemSizes:
+        return SystemSizes(self.num_states, self.num_parameters, len(self.constant_names),
+                           self.num_observables, self.num_drivers)
+
+    @property
+    def initial_values(self) -> Dict[str, float]:
+        return dict(self._state_defaults)
+
+    @property
+    def parameters(self) -> Dict[str, float]:
+        return dict(self._parameter_defaults)
+
+    @property
+    def constants(self) -> Dict[str, float]:
+        return dict(self._constants)
+
+    def set_constants(self, updates: Mapping[str, float]) -> None:
+        """Change constant values (they are compiled in, so this rebuilds)."""
+        for k, v in updates.items():
+            if k not in self._constants:
+                raise KeyError(f"'{k}' is not a constant of this system")
+            self._constants[k] = float(v)
+
+    def update_defaults(self, initial_values=None, parameters=None) -> None:
+        for k, v in (initial_values or {}).items():
+            if k not in self._state_defaults:
+                raise KeyError(k)
+            self._state_defaults[k] = float(v)
+        for k, v in (parameters or {}).items():
+            if k not in self._parameter_defaults:
+                raise KeyError(k)
+            self._parameter_defaults[k] = float(v)
+
+    @property
+    def fn_hash(self) -> str:
+        h = hashlib.sha256()
+        for a in self.assignments:
+            h.update(f"{a.lhs}={sp.srepr(a.rhs)}|{a.kind};".encode())
+        for grp in (self.state_names, self.parameter_names, self.constant_names,
+                    self.driver_names, self.observable_names):
+            h.update(("," .join(grp) + "/").encode())
+        for k in sorted(self._constants):
+            h.update(f"{k}:{self._constants[k]!r};".encode())
+        for k in sorted(self.user_functions):
+            h.update(f"{k}:{self.user_functions[k]};".encode())
+        return h.hexdigest()
+
+    # -- expressions -------------------------------------------------------
+    def derivative_exprs(self) -> List[sp.Expr]:
+        """Fully substituted f(t, y) expressions, one per state."""
+        env: Dict[sp.Symbol, sp.Expr] = {}
+        for a in self.assignments:
+            env[a.lhs] = a.rhs.xreplace(env)
+        return [env[d] for d in self.derivative_symbols]
+
+    def observable_exprs(self) -> List[sp.Expr]:
+        env: Dict[sp.Symbol, sp.Expr] = {}
+        for a in self.assignments:
+            env[a.lhs] = a.rhs.xreplace(env)
+        return [env[o] for o in self.observable_symbols]
+
+    def numpy_rhs(self):
+        """Vectorised NumPy callables (rhs, observables, jvp) for CPU checks."""
+        args = (self.state_symbols, self.parameter_symbols, self.driver_symbols, TIME_SYMBOL)
+        const_sub = {sp.Symbol(k, real=True): v for k, v in self._constants.items()}
+        f = [e.xreplace(const_sub) for e in self.derivative_exprs()]
+        o = [e.xreplace(const_sub) for e in self.observable_exprs()]
+        v = [sp.Symbol(f"_v{i}", real=True) for i in range(self.num_states)]
+        jv = [sum(sp.diff(fi, yj) * vj for yj, vj in zip(self.state_symbols, v)) for fi in f]
+        mods = [{"Heaviside": lambda x: np.heaviside(x, 0.5)}, "numpy"]
+        rhs = sp.lambdify(args, f, modules=mods)
+        obs = sp.lambdify(args, o, modules=mods)
+        jvp = sp.lambdify(args + (v,), jv, modules=mods)
+        return rhs, obs, jvp
+
+    def __repr__(self) -> str:
+        return (f"SymbolicODE({self.name!r}, states={self.state_names}, "
+                f"parameters={self.parameter_names}, drivers={self.driver_names}, "
+                f"observables={self.observable_names})")
+
+
+def _parse_rhs(rhs: str, local: dict) -> sp.Expr:
+    text = rhs.replace("^", "**")
+    try:
+        expr = sp.parse_expr(text, local_dict=local, evaluate=True)
+    except Exception as exc:  # sympy raises many types
+        raise ValueError(f"Could not parse expression '{rhs}': {exc}") from exc
+    return sp.sympify(expr)
+
+
+def _topological(assignments: List[Assignment]) -> List[Assignment]:
+    by_lhs = {a.lhs: a for a in assignments}
+    order: List[Assignment] = []
+    state: Dict[sp.Symbol, int] = {}
+
+    def visit(sym, stack):
+        st = state.get(sym, 0)
+        if st == 2:
+            return
+        if st == 1:
+            raise ValueError(f"Circular definition involving '{sym}'")
+        state[sym] = 1
+        for dep in by_lhs[sym].rhs.free_symbols:
+            if dep in by_lhs and dep != sym:
+                visit(dep, stack + [sym])
+        state[sym] = 2
+        order.append(by_lhs[sym])
+
+    for a in assignments:
+        visit(a.lhs, [])
+    return order
+
+
+def create_ODE_system(dxdt, precision=np.float32, states=None, observables=None,
+                      parameters=None, constants=None, drivers=None,
+                      user_functions=None, user_function_derivatives=None,
+                      name=None, strict=False, **kwargs) -> SymbolicODE:
+    """Build a :class:`SymbolicODE` (reference: symbolicODE.py:102-200).
+
+    ``user_functions`` maps a function name used in the equations to the HIP
+    C++ expression template that implements it (``"{0}"`` stands for the
+    first argument), e.g. ``{"sq": "({0})*({0})"}``.
+    """
+    return SymbolicODE.create(dxdt=dxdt, states=states, observables=observables,
+                              parameters=parameters, constants=constants,
+                              drivers=drivers, user_functions=user_functions,
+                              name=name, precision=precision, strict=strict)

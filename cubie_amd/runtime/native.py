@@ -1,0 +1,186 @@
+"""ctypes binding of the C ABI in include/cubie_hip.h.
+
+The shared library ``libcubie_hip.so`` is built in-tree by
+:func:`build_runtime` (also run by ``__graft_entry__.build``). Loading it
+always imports torch first, so that the HIP runtime torch ships and the one
+this library links against are the same loaded copy (they share the
+``libamdhip64.so.7`` soname); device buffers allocated through torch are then
+valid for our launches. Missing or unloadable native code is an error: there
+is no CPU fallback on the product path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import threading
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(_HERE)
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "csrc", "libcubie_hip.so")
+RUNTIME_SRC = os.path.join(PKG, "csrc", "cubie_hip_runtime.cpp")
+INCLUDE_DIR = os.path.join(ROOT, "include")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+class CubieNativeError(RuntimeError):
+    pass
+
+
+class SolveArgs(ctypes.Structure):
+    """Mirror of ``cubie_solve_args`` (include/cubie_hip.h)."""
+
+    _fields_ = [
+        ("inits", ctypes.c_void_p),
+        ("params", ctypes.c_void_p),
+        ("driver_coeffs", ctypes.c_void_p),
+        ("state_out", ctypes.c_void_p),
+        ("observables_out", ctypes.c_void_p),
+        ("state_summ_out", ctypes.c_void_p),
+        ("obs_summ_out", ctypes.c_void_p),
+        ("counters_out", ctypes.c_void_p),
+        ("status_out", ctypes.c_void_p),
+        ("rtol_per_run", ctypes.c_void_p),
+        ("duration_per_run", ctypes.c_void_p),
+        ("save_times", ctypes.c_void_p),
+        ("work_counter", ctypes.c_void_p),
+        ("checkpoints", ctypes.c_void_p),
+        ("checkpoint_stride", ctypes.c_longlong),
+        ("duration", ctypes.c_double),
+        ("warmup", ctypes.c_double),
+        ("t0", ctypes.c_double),
+        ("save_stop", ctypes.c_double),
+        ("summary_stop", ctypes.c_double),
+        ("n_runs", ctypes.c_longlong),
+        ("out_rows", ctypes.c_longlong),
+        ("summary_rows", ctypes.c_longlong),
+        ("chunk_runs", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+    ]
+
+
+def build_runtime(force: bool = False) -> str:
+    """Compile ``libcubie_hip.so`` (host code: hiprtc + module launch)."""
+    if not force and os.path.exists(LIB_PATH) and os.path.getmtime(LIB_PATH) >= max(
+            os.path.getmtime(RUNTIME_SRC), os.path.getmtime(os.path.join(INCLUDE_DIR, "cubie_hip.h"))):
+        return LIB_PATH
+    tmp = LIB_PATH + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-D__HIP_PLATFORM_AMD__",
+           f"-I{ROCM}/include", f"-I{INCLUDE_DIR}", RUNTIME_SRC, "-o", tmp,
+           f"-L{ROCM}/lib", "-lhiprtc", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise CubieNativeError("building libcubie_hip.so failed:\n" + res.stderr)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+_lib: Optional[ctypes.CDLL] = None
+_lock = threading.Lock()
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        import torch  # noqa: F401  (share torch's HIP runtime, see module doc)
+        if not os.path.exists(LIB_PATH):
+            raise CubieNativeError(
+                f"{LIB_PATH} is missing; build it with cubie_amd.runtime.native.build_runtime() "
+                "or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.cubie_hip_abi_version.restype = ctypes.c_int
+        L.cubie_hip_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.cubie_hip_last_error.restype = ctypes.c_int
+        L.cubie_hip_compile.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.POINTER(ctypes.c_void_p),
+                                        ctypes.POINTER(ctypes.c_size_t),
+                                        ctypes.c_char_p, ctypes.c_size_t]
+        L.cubie_hip_compile.restype = ctypes.c_int
+        L.cubie_hip_free_code.argtypes = [ctypes.c_void_p]
+        L.cubie_hip_load.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p,
+                                     ctypes.POINTER(ctypes.c_void_p)]
+        L.cubie_hip_load.restype = ctypes.c_int
+        L.cubie_hip_unload.argtypes = [ctypes.c_void_p]
+        L.cubie_hip_unload.restype = ctypes.c_int
+        L.cubie_hip_kernel_info.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 4
+        L.cubie_hip_kernel_info.restype = ctypes.c_int
+        L.cubie_hip_launch.argtypes = [ctypes.c_void_p, ctypes.POINTER(SolveArgs), ctypes.c_uint,
+                                       ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p]
+        L.cubie_hip_launch.restype = ctypes.c_int
+        L.cubie_hip_launch_timed.argtypes = [ctypes.c_void_p, ctypes.POINTER(SolveArgs),
+                                             ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                             ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.cubie_hip_launch_timed.restype = ctypes.c_int
+        if L.cubie_hip_abi_version() != 1:
+            raise CubieNativeError("libcubie_hip.so ABI version mismatch; rebuild it")
+        _lib = L
+        return L
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(8192)
+    lib().cubie_hip_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise CubieNativeError(f"{what} failed ({rc}): {last_error()}")
+
+
+def compile_source(source: str, options, name: str = "cubie_kernel.hip") -> bytes:
+    L = lib()
+    code = ctypes.c_void_p()
+    size = ctypes.c_size_t()
+    log = ctypes.create_string_buffer(1 << 16)
+    rc = L.cubie_hip_compile(source.encode(), name.encode(), "\n".join(options).encode(),
+                             ctypes.byref(code), ctypes.byref(size), log, len(log))
+    if rc != 0:
+        raise CubieNativeError("hiprtc compilation failed:\n" + last_error())
+    try:
+        return ctypes.string_at(code, size.value)
+    finally:
+        L.cubie_hip_free_code(code)
+
+
+class Module:
+    """A loaded code object with one resolved kernel."""
+
+    def __init__(self, code: bytes, kernel_name: str):
+        L = lib()
+        self._code = code  # keep alive
+        handle = ctypes.c_void_p()
+        check(L.cubie_hip_load(code, len(code), kernel_name.encode(), ctypes.byref(handle)),
+              "cubie_hip_load")
+        self.handle = handle
+        self.kernel_name = kernel_name
+
+    def info(self):
+        regs, lds, priv, maxt = (ctypes.c_int() for _ in range(4))
+        check(lib().cubie_hip_kernel_info(self.handle, ctypes.byref(regs), ctypes.byref(lds),
+                                          ctypes.byref(priv), ctypes.byref(maxt)), "kernel_info")
+        return {"vgprs": regs.value, "lds_bytes": lds.value, "scratch_bytes": priv.value,
+                "max_threads": maxt.value}
+
+    def launch(self, args: SolveArgs, grid: int, block: int, stream: int = 0, lds: int = 0) -> None:
+        check(lib().cubie_hip_launch(self.handle, ctypes.byref(args), grid, block, lds,
+                                     ctypes.c_void_p(stream)), "cubie_hip_launch")
+
+    def launch_timed(self, args: SolveArgs, grid: int, block: int, stream: int = 0, lds: int = 0) -> float:
+        ms = ctypes.c_float()
+        check(lib().cubie_hip_launch_timed(self.handle, ctypes.byref(args), grid, block, lds,
+                                           ctypes.c_void_p(stream), ctypes.byref(ms)),
+              "cubie_hip_launch_timed")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) is not None and _lib is not None:
+                _lib.cubie_hip_unload(self.handle)
+        except Exception:
+            pass

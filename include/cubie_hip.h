@@ -1,0 +1,154 @@
+/*
+ * cubie_hip.h - C ABI of the MI355X batch ODE integration engine.
+ *
+ * The reference (ccam80/cubie 0.3.1) is pure Python + Numba: its integration
+ * path sits behind BatchSolverKernel.run() (src/cubie/batchsolving/
+ * BatchSolverKernel.py:525-604), which launches the single compiled
+ * `integration_kernel` (BatchSolverKernel.py:887-998) with the argument list
+ *   (inits, params, d_coefficients, state_output, observables_output,
+ *    state_summaries_output, observables_summaries_output,
+ *    iteration_counters_output, status_codes_output,
+ *    duration, warmup, t0, save_stop, summary_stop, n_runs).
+ * This library replaces the Numba JIT + launch underneath that call: a
+ * generated HIP C++ translation unit (system functions + the hand-written
+ * loop kernel, specialised by compile-time constants) is compiled with hiprtc
+ * for gfx950 and launched on a caller-supplied stream. The Python package
+ * (cubie_amd, mirroring cubie's Solver/solve_ivp API) is the only caller and
+ * binds these symbols with ctypes.
+ *
+ * Conventions: every entry point returns 0 on success and a negative code on
+ * failure (CUBIE_E_*); cubie_hip_last_error() returns the message of the last
+ * failure on the calling thread. All device pointers are caller-owned.
+ * This header is also compiled into the device translation unit, so the
+ * shared structs below use only builtin C types.
+ */
+#ifndef CUBIE_HIP_H
+#define CUBIE_HIP_H
+
+#define CUBIE_ABI_VERSION 1
+
+/* Error codes */
+#define CUBIE_OK 0
+#define CUBIE_E_INVALID (-1)
+#define CUBIE_E_COMPILE (-2)
+#define CUBIE_E_HIP (-3)
+#define CUBIE_E_NOT_FOUND (-4)
+#define CUBIE_E_IO (-5)
+
+/*
+ * Launch arguments of one integration chunk, passed by value to the kernel.
+ * Mirrors the reference integration_kernel argument list
+ * (BatchSolverKernel.py:887-903) plus the per-run extensions the batch
+ * configurations need (per-run rtol / end time / save-time table) and the
+ * persistent-kernel work queue.
+ *
+ * Array layouts (the run axis is always fastest, so lanes of a wavefront
+ * touch consecutive addresses):
+ *   inits           [n_states][n_runs]              precision
+ *   params          [n_params][n_runs]              precision
+ *   driver_coeffs   [segments][n_drivers][order+1]  precision
+ *   state_out       [rows][n_saved_states + save_time][n_runs]
+ *   observables_out [rows][n_saved_observables][n_runs]
+ *   state_summ_out  [summary_rows][n_summarised_states * height][n_runs]
+ *   obs_summ_out    [summary_rows][n_summarised_observables * height][n_runs]
+ *   counters_out    [rows][4][n_runs] int32  (newton, krylov, steps, rejects)
+ *   status_out      [n_runs] int32
+ *   rtol_per_run    [n_runs] precision or null
+ *   t_end_per_run   [n_runs] double (duration per run) or null
+ *   save_times      [rows] double, shared save table, or null
+ */
+typedef struct cubie_solve_args {
+    const void* inits;
+    const void* params;
+    const void* driver_coeffs;
+    void* state_out;
+    void* observables_out;
+    void* state_summ_out;
+    void* obs_summ_out;
+    int* counters_out;
+    int* status_out;
+    const void* rtol_per_run;
+    const double* duration_per_run;
+    const double* save_times;
+    /* persistent kernel work queue: one int, zeroed by the caller before launch */
+    int* work_counter;
+    /* adjoint checkpoint buffer (unused by forward-only kernels) */
+    void* checkpoints;
+    long long checkpoint_stride;
+    double duration;
+    double warmup;
+    double t0;
+    double save_stop;
+    double summary_stop;
+    long long n_runs;
+    long long out_rows;        /* rows allocated in state/observable outputs */
+    long long summary_rows;    /* rows allocated in summary outputs */
+    int chunk_runs;            /* runs taken per queue grab (persistent kernel) */
+    int reserved;
+} cubie_solve_args;
+
+#ifndef CUBIE_DEVICE_COMPILE
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cubie_module* cubie_module_t;
+
+/* ABI version compiled into the library (CUBIE_ABI_VERSION). */
+int cubie_hip_abi_version(void);
+
+/* Message of the last failure on this thread; returns the message length. */
+int cubie_hip_last_error(char* buf, size_t len);
+
+/*
+ * Compile a HIP C++ translation unit with hiprtc for `arch` (e.g. "gfx950")
+ * and write the code object to *code / *code_size (malloc'd, free with
+ * cubie_hip_free_code). `options` is a '\n'-separated hiprtc option list.
+ * Replaces the reference's Numba JIT of the integration kernel
+ * (CUDAFactory.py:365-700 build cascade, BatchSolverKernel.build_kernel
+ * BatchSolverKernel.py:850-1010). Needs no GPU.
+ */
+int cubie_hip_compile(const char* source, const char* name, const char* options,
+                      void** code, size_t* code_size, char* log, size_t log_len);
+void cubie_hip_free_code(void* code);
+
+/*
+ * Load a code object on the current device and resolve `kernel_name`.
+ * The reference's equivalent is the dispatcher's first-call load of the
+ * compiled kernel (BatchSolverKernel.py:1000-1010 cache attach).
+ */
+int cubie_hip_load(const void* code, size_t code_size, const char* kernel_name,
+                   cubie_module_t* module);
+int cubie_hip_unload(cubie_module_t module);
+
+/* Kernel attributes used by the launch planner: registers, LDS, spills. */
+int cubie_hip_kernel_info(cubie_module_t module, int* num_regs,
+                          int* shared_bytes, int* private_bytes,
+                          int* max_threads);
+
+/*
+ * Launch one chunk: grid x block threads on `stream` (hipStream_t, may be
+ * null for the null stream) with `dynamic_lds` bytes of dynamic LDS.
+ * Replaces the kernel launch at BatchSolverKernel.py:741-762.
+ */
+int cubie_hip_launch(cubie_module_t module, const cubie_solve_args* args,
+                     unsigned grid, unsigned block, unsigned dynamic_lds,
+                     void* stream);
+
+/*
+ * Launch and time one chunk with a pair of hipEvents recorded on `stream`
+ * around the launch; *elapsed_ms receives the kernel time after the stream
+ * completes. Mirrors the reference's per-chunk kernel CUDAEvent pair
+ * (BatchSolverKernel.py:738-763, time_logger.py:64-157).
+ */
+int cubie_hip_launch_timed(cubie_module_t module, const cubie_solve_args* args,
+                           unsigned grid, unsigned block, unsigned dynamic_lds,
+                           void* stream, float* elapsed_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUBIE_DEVICE_COMPILE */
+
+#endif /* CUBIE_HIP_H */

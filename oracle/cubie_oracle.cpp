@@ -1,0 +1,970 @@
+// CPU oracle for the batch ODE integration path.
+//
+// TEST INFRASTRUCTURE ONLY. This file restates, on the CPU, the reference's
+// per-system integration algorithm so that the HIP kernels can be checked
+// against it. Only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline leg may load it; the product path never does.
+//
+// Restated reference functions (ccam80/cubie 0.3.1, paths under src/cubie/):
+//   loop ............ integrators/loops/ode_loop.py:562-984 (loop_fn)
+//   ERK step ........ integrators/algorithms/generic_erk.py:359-564
+//   tableau helpers . integrators/algorithms/base_algorithm_step.py:290-560
+//   PI controller ... integrators/step_control/adaptive_PI_controller.py:150-268
+//   I controller .... integrators/step_control/adaptive_I_controller.py
+//   PID controller .. integrators/step_control/adaptive_PID_controller.py:156-284
+//   Gustafsson ...... integrators/step_control/gustafsson_controller.py:173-285
+//   clamp ........... _utils.py:315-342
+//   save state ...... outputhandling/save_state.py (save_state_func)
+//   summaries ....... outputhandling/update_summaries.py, save_summaries.py,
+//                     outputhandling/summarymetrics/*.py
+//   driver eval ..... array_interpolator.py:463-500 (evaluate_all)
+//   status codes .... result_codes.py:22-58
+// The implicit families live in oracle_implicit.hpp (included below).
+//
+// Every function is templated on the working precision T (float or double);
+// time is accumulated in double exactly as the reference loop does.
+// Built with -ffp-contract=off so that no multiply-add is fused.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <limits>
+
+#include "systems.hpp"
+
+namespace oracle {
+
+// Status bits, result_codes.py:22-58
+enum : int32_t {
+    ST_SUCCESS = 0,
+    ST_NEWTON_MAX_ITERS = 2,
+    ST_LINEAR_MAX_ITERS = 4,
+    ST_STEP_TOO_SMALL = 8,
+    ST_STAGNATION = 64,
+    ST_BICGSTAB_BREAKDOWN = 128,
+    ST_NEWTON_DIVERGENCE = 256,
+};
+
+enum Family { FAM_ERK = 0, FAM_DIRK = 1, FAM_FIRK = 2, FAM_ROSENBROCK = 3 };
+enum Controller { CTRL_FIXED = 0, CTRL_I = 1, CTRL_PI = 2, CTRL_PID = 3, CTRL_GUSTAFSSON = 4 };
+
+// Summary metric ids (registration order of the reference's built-ins,
+// outputhandling/summarymetrics/__init__.py).
+enum Metric {
+    M_MEAN = 0, M_MAX = 1, M_RMS = 2, M_PEAKS = 3, M_STD = 4, M_MIN = 5,
+    M_MAX_MAGNITUDE = 6, M_EXTREMA = 7, M_NEGATIVE_PEAKS = 8,
+    M_MEAN_STD_RMS = 9, M_MEAN_STD = 10, M_STD_RMS = 11,
+    M_DXDT_MAX = 12, M_DXDT_MIN = 13, M_DXDT_EXTREMA = 14,
+    M_D2XDT2_MAX = 15, M_D2XDT2_MIN = 16, M_D2XDT2_EXTREMA = 17,
+};
+
+static const int MAXS = 16;   // max stages
+static const int MAXN = 64;   // max states handled by the oracle
+
+extern "C" {
+
+// Mirror of oracle.OracleConfig (ctypes). Plain data only.
+struct OracleConfig {
+    int32_t precision_bits;
+    int32_t system_id;
+    int32_t n_states, n_params, n_obs, n_drivers;
+    const double* constants;
+    int32_t n_constants;
+    // algorithm
+    int32_t family;
+    int32_t stages;
+    int32_t order;
+    int32_t has_error;
+    const double* a;       // stages*stages, row-major
+    const double* b;
+    const double* b_hat;   // may be null
+    const double* c;
+    // Rosenbrock extras
+    const double* rb_C;        // stages*stages, row-major
+    const double* rb_gamma_stages; // stages
+    double rb_gamma;
+    // controller
+    int32_t controller;
+    double dt, dt_min, dt_max;
+    double kp, ki, kd;
+    double min_gain, max_gain, safety, deadband_min, deadband_max;
+    int32_t newton_target_iters;
+    const double* atol;   // n_states
+    const double* rtol;   // n_states
+    // loop timing
+    double save_every, summarise_every, sample_summaries_every;
+    int32_t save_last, save_regularly, summarise_regularly, samples_per_summary;
+    // outputs
+    int32_t save_state, save_obs, save_time, save_counters;
+    int32_t summarise_state, summarise_obs;
+    int32_t n_saved_states, n_saved_obs, n_summ_states, n_summ_obs;
+    const int32_t* saved_state_idx;
+    const int32_t* saved_obs_idx;
+    const int32_t* summ_state_idx;
+    const int32_t* summ_obs_idx;
+    int32_t n_metrics;
+    const int32_t* metric_ids;
+    const int32_t* metric_params;
+    int32_t summ_buf_height;   // per variable
+    int32_t summ_out_height;   // per variable
+    int32_t n_out_rows;
+    int32_t n_summary_rows;
+    // drivers
+    int32_t drv_order, drv_segments, drv_wrap, drv_pad_clamped;
+    double drv_t0, drv_dt;
+    const double* drv_coeffs;  // (segments, n_drivers, order+1)
+    // run times (already cast to the working precision on the host)
+    double duration, warmup, t0, save_stop, summary_stop;
+    // implicit solver settings
+    int32_t newton_max_iters;
+    int32_t krylov_max_iters;
+    int32_t linear_solver;     // 0 minimal residual, 1 steepest descent, 2 bicgstab
+    int32_t preconditioner_order;
+    double newton_atol_s, newton_rtol_s;   // scalar tolerances (uniform)
+    double krylov_atol_s, krylov_rtol_s;
+    double krylov_residual_reduction;
+    double krylov_floor;
+    // callbacks (SYS_CALLBACK)
+    rhs_cb_t rhs_cb;
+    obs_cb_t obs_cb;
+    jvp_cb_t jvp_cb;
+};
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------
+// Driver evaluation, array_interpolator.py:463-500
+// --------------------------------------------------------------------------
+template <typename T>
+struct DriverEval {
+    const OracleConfig* cfg;
+    T resolution, inv_resolution, evaluation_start;
+    void init(const OracleConfig* c) {
+        cfg = c;
+        resolution = T(c->drv_dt);
+        inv_resolution = T(T(1.0) / resolution);
+        T start_time = T(c->drv_t0);
+        evaluation_start = T(start_time - (c->drv_pad_clamped ? resolution : T(0)));
+    }
+    void eval(T time, T* out) const {
+        const int nd = cfg->n_drivers;
+        if (nd == 0) return;
+        T scaled = (time - evaluation_start) * inv_resolution;
+        T scaled_floor = T(std::floor(scaled));
+        int32_t idx = int32_t(scaled_floor);
+        int32_t nseg = cfg->drv_segments;
+        int32_t seg;
+        T tau;
+        bool in_range;
+        if (cfg->drv_wrap) {
+            // Python modulo semantics: result takes the sign of the divisor
+            seg = idx % nseg;
+            if (seg < 0) seg += nseg;
+            tau = T(scaled - scaled_floor);
+            in_range = true;
+        } else {
+            in_range = (scaled >= T(0)) && (double(scaled) <= double(nseg));
+            seg = idx < 0 ? 0 : idx;
+            seg = seg >= nseg ? nseg - 1 : seg;
+            tau = T(scaled - T(seg));
+        }
+        const int order = cfg->drv_order;
+        for (int i = 0; i < nd; ++i) {
+            T acc = T(0);
+            for (int k = order; k >= 0; --k) {
+                acc = acc * tau + T(cfg->drv_coeffs[(size_t(seg) * nd + i) * (order + 1) + k]);
+            }
+            out[i] = in_range ? acc : T(0);
+        }
+    }
+};
+
+// --------------------------------------------------------------------------
+// Problem context shared by steps
+// --------------------------------------------------------------------------
+template <typename T>
+struct Problem {
+    const OracleConfig* cfg;
+    SystemDesc sys;
+    DriverEval<T> drv;
+    int n;
+    void rhs(const T* y, const T* p, const T* d, T* obs, T* dy, T t) const {
+        sys_rhs<T>(sys, y, p, d, obs, dy, t);
+    }
+    void observables(const T* y, const T* p, const T* d, T* obs, T t) const {
+        if (sys.n_obs > 0) sys_observables<T>(sys, y, p, d, obs, t);
+    }
+    void jvp(const T* y, const T* p, const T* d, const T* v, T* jv, T t) const {
+        sys_jvp<T>(sys, y, p, d, v, jv, t);
+    }
+    void drivers(T t, T* out) const { drv.eval(t, out); }
+};
+
+// --------------------------------------------------------------------------
+// ERK step, generic_erk.py:359-564 (streamed accumulators)
+// --------------------------------------------------------------------------
+template <typename T>
+struct ERKStepper {
+    const Problem<T>* P;
+    int s, n;
+    T A[MAXS][MAXS];  // a[row][col]
+    T b[MAXS], c[MAXS], d[MAXS];
+    bool has_error, fsal, multistage, accumulates_output, accumulates_error;
+    int b_row, b_hat_row;  // -1 when absent
+    // persistent
+    T stage_rhs[MAXN];
+
+    static int find_row(const OracleConfig* cfg, const double* w) {
+        // base_algorithm_step.py _find_matching_row: last row within 1e-15
+        if (!w) return -1;
+        int match = -1;
+        for (int r = 0; r < cfg->stages; ++r) {
+            bool ok = true;
+            for (int i = 0; i < cfg->stages; ++i) {
+                if (std::fabs(cfg->a[r * cfg->stages + i] - w[i]) > 1e-15) { ok = false; break; }
+            }
+            if (ok) match = r;
+        }
+        return match;
+    }
+
+    void init(const Problem<T>* p) {
+        P = p;
+        const OracleConfig* cfg = p->cfg;
+        s = cfg->stages;
+        n = p->n;
+        for (int i = 0; i < s; ++i) {
+            for (int j = 0; j < s; ++j) A[i][j] = T(cfg->a[i * s + j]);
+            b[i] = T(cfg->b[i]);
+            c[i] = T(cfg->c[i]);
+        }
+        has_error = cfg->has_error != 0;
+        for (int i = 0; i < s; ++i)
+            d[i] = has_error ? T(cfg->b[i] - cfg->b_hat[i]) : T(0);
+        // first_same_as_last, base_algorithm_step.py:470-490
+        bool a_last_eq_b = true;
+        for (int i = 0; i < s; ++i)
+            if (cfg->a[(s - 1) * s + i] != cfg->b[i]) a_last_eq_b = false;
+        bool first_row_zero = true;
+        for (int i = 0; i < s; ++i)
+            if (cfg->a[i] != 0.0) first_row_zero = false;
+        fsal = cfg->c[0] == 0.0 && cfg->c[s - 1] == 1.0 && a_last_eq_b && first_row_zero;
+        multistage = s > 1;
+        b_row = find_row(cfg, cfg->b);
+        b_hat_row = has_error ? find_row(cfg, cfg->b_hat) : -1;
+        accumulates_output = b_row < 0;
+        accumulates_error = b_hat_row < 0;
+        for (int i = 0; i < MAXN; ++i) stage_rhs[i] = T(0);
+    }
+
+    int32_t step(const T* state, T* proposed_state, const T* params,
+                 T* drivers, T* proposed_drivers, T* observables,
+                 T* proposed_observables, T* error, T dt, T time,
+                 bool first_step, bool accepted_flag, int32_t* counters) {
+        const int acc_len = (s - 1) * n;
+        T stage_acc[MAXS * MAXN];
+        T current_time = time;
+        T end_time = current_time + dt;
+        for (int i = 0; i < n; ++i) {
+            if (accumulates_output) proposed_state[i] = T(0);
+            if (has_error && accumulates_error) error[i] = T(0);
+        }
+        bool use_cached = false;
+        if (fsal && multistage && !first_step) use_cached = accepted_flag;
+        if (!multistage || !use_cached) {
+            P->rhs(state, params, drivers, observables, stage_rhs, current_time);
+        }
+        for (int i = 0; i < n; ++i) {
+            T inc = stage_rhs[i];
+            if (accumulates_output) proposed_state[i] += b[0] * inc;
+            if (has_error && accumulates_error) error[i] = error[i] + d[0] * inc;
+        }
+        for (int i = 0; i < acc_len; ++i) stage_acc[i] = T(0);
+        for (int prev = 0; prev < s - 1; ++prev) {
+            int stage_offset = prev * n;
+            int stage_idx = prev + 1;
+            for (int succ = 0; succ < s - 1; ++succ) {
+                T coeff = A[succ + 1][prev];
+                int row_offset = succ * n;
+                for (int i = 0; i < n; ++i) stage_acc[row_offset + i] += coeff * stage_rhs[i];
+            }
+            T dt_stage = dt * c[stage_idx];
+            T stage_time = current_time + dt_stage;
+            for (int i = 0; i < n; ++i)
+                stage_acc[stage_offset + i] = stage_acc[stage_offset + i] * dt + state[i];
+            P->drivers(stage_time, proposed_drivers);
+            P->observables(&stage_acc[stage_offset], params, proposed_drivers,
+                           proposed_observables, stage_time);
+            P->rhs(&stage_acc[stage_offset], params, proposed_drivers,
+                   proposed_observables, stage_rhs, stage_time);
+            T sw = b[stage_idx], ew = d[stage_idx];
+            for (int i = 0; i < n; ++i) {
+                if (accumulates_output) proposed_state[i] += sw * stage_rhs[i];
+                if (has_error && accumulates_error) error[i] += ew * stage_rhs[i];
+            }
+        }
+        if (b_row >= 0)
+            for (int i = 0; i < n; ++i) proposed_state[i] = stage_acc[(b_row - 1) * n + i];
+        if (b_hat_row >= 0)
+            for (int i = 0; i < n; ++i) error[i] = stage_acc[(b_hat_row - 1) * n + i];
+        for (int i = 0; i < n; ++i) {
+            if (accumulates_output) proposed_state[i] = proposed_state[i] * dt + state[i];
+            if (has_error) {
+                if (accumulates_error) error[i] *= dt;
+                else error[i] = proposed_state[i] - error[i];
+            }
+        }
+        P->drivers(end_time, proposed_drivers);
+        P->observables(proposed_state, params, proposed_drivers, proposed_observables, end_time);
+        counters[0] = 0;
+        counters[1] = 0;
+        return ST_SUCCESS;
+    }
+};
+
+// --------------------------------------------------------------------------
+// Step-size controllers
+// --------------------------------------------------------------------------
+template <typename T>
+static inline T clampv(T v, T lo, T hi) { return std::max(lo, std::min(v, hi)); }
+
+template <typename T>
+static inline T tpow(T x, T e) { return std::pow(x, e); }
+
+template <typename T>
+struct ControllerState {
+    const OracleConfig* cfg;
+    int n;
+    T atol[MAXN], rtol[MAXN];
+    T history[2];
+    T kp, ki, kd_e, expo, safety, min_gain, max_gain, dt_min, dt_max;
+    T deadband_min, deadband_max, inv_n, gain_numerator;
+    bool deadband_disabled;
+    int newton_target;
+
+    void init(const OracleConfig* c, int n_) {
+        cfg = c;
+        n = n_;
+        for (int i = 0; i < n; ++i) { atol[i] = T(c->atol[i]); rtol[i] = T(c->rtol[i]); }
+        history[0] = history[1] = T(0);
+        const int k = c->order;
+        safety = T(c->safety);
+        min_gain = T(c->min_gain);
+        max_gain = T(c->max_gain);
+        dt_min = T(c->dt_min);
+        dt_max = T(c->dt_max);
+        deadband_min = T(c->deadband_min);
+        deadband_max = T(c->deadband_max);
+        deadband_disabled = (deadband_min == T(1)) && (deadband_max == T(1));
+        inv_n = T(1.0 / n);
+        newton_target = c->newton_target_iters;
+        if (c->controller == CTRL_PI) {
+            kp = T(c->kp / ((k + 1) * 2));
+            ki = T(c->ki / ((k + 1) * 2));
+        } else if (c->controller == CTRL_PID) {
+            kp = T(c->kp / (2 * (k + 1)));
+            ki = T(c->ki / (2 * (k + 1)));
+            kd_e = T(c->kd / (2 * (k + 1)));
+        }
+        expo = T(1.0 / (2 * (k + 1)));
+        gain_numerator = T(1 + 2 * newton_target) * safety;
+    }
+
+    T norm(const T* state, const T* state_prev, const T* error) const {
+        T nrm2 = T(0);
+        for (int i = 0; i < n; ++i) {
+            T ei = std::max(std::fabs(error[i]), T(1e-16));
+            T tol = atol[i] + rtol[i] * std::max(std::fabs(state[i]), std::fabs(state_prev[i]));
+            T ratio = ei / tol;
+            nrm2 += ratio * ratio;
+        }
+        nrm2 = nrm2 * inv_n;
+        if (std::isnan(nrm2) || std::isinf(nrm2)) nrm2 = T(1e16);
+        return nrm2;
+    }
+
+    T deadband(T gain) const {
+        if (!deadband_disabled) {
+            bool within = (gain >= deadband_min) && (gain <= deadband_max);
+            gain = within ? T(1) : gain;
+        }
+        return gain;
+    }
+
+    // Returns status; writes dt and accept.
+    int32_t control(T* dt, const T* state, const T* state_prev, const T* error,
+                    int32_t niters, bool truncated, int32_t* accept_out) {
+        T nrm2 = norm(state, state_prev, error);
+        bool accept = nrm2 <= T(1);
+        *accept_out = accept ? 1 : 0;
+        T gain;
+        bool freeze = accept && truncated;
+        bool commit = accept && !truncated;
+        T cur = dt[0];
+        T dt_new_raw;
+        switch (cfg->controller) {
+        case CTRL_I: {
+            T gaintmp = T(safety * tpow<T>(nrm2, -expo));
+            gain = clampv(gaintmp, min_gain, max_gain);
+            gain = deadband(gain);
+            gain = accept ? gain : std::min(gain, safety);
+            dt_new_raw = cur * gain;
+            dt[0] = freeze ? cur : clampv(dt_new_raw, dt_min, dt_max);
+            break;
+        }
+        case CTRL_PI: {
+            T err_prev = history[0];
+            T pgain = T(tpow<T>(nrm2, -kp));
+            T err_source = err_prev > T(0) ? err_prev : nrm2;
+            T igain = T(tpow<T>(err_source, -ki));
+            T gain_new = safety * pgain * igain;
+            gain = clampv(gain_new, min_gain, max_gain);
+            gain = deadband(gain);
+            gain = accept ? gain : std::min(gain, safety);
+            dt_new_raw = cur * gain;
+            dt[0] = freeze ? cur : clampv(dt_new_raw, dt_min, dt_max);
+            history[0] = commit ? nrm2 : err_prev;
+            break;
+        }
+        case CTRL_PID: {
+            T err_prev = history[0], err_prev_prev = history[1];
+            T eps = err_prev > T(0) ? err_prev : nrm2;
+            T epps = err_prev_prev > T(0) ? err_prev_prev : eps;
+            T gain_new = T(safety * tpow<T>(nrm2, -kp) * tpow<T>(eps, -ki) * tpow<T>(epps, -kd_e));
+            gain = clampv(gain_new, min_gain, max_gain);
+            gain = deadband(gain);
+            gain = accept ? gain : std::min(gain, safety);
+            dt_new_raw = cur * gain;
+            dt[0] = freeze ? cur : clampv(dt_new_raw, dt_min, dt_max);
+            history[1] = commit ? err_prev : err_prev_prev;
+            history[0] = commit ? nrm2 : err_prev;
+            break;
+        }
+        case CTRL_GUSTAFSSON: {
+            T dt_prev = std::max(history[0], T(1e-16));
+            T err_prev = std::max(history[1], T(1e-16));
+            T denom = T(niters + 2 * newton_target);
+            T tmp = gain_numerator / denom;
+            T fac = safety < tmp ? safety : tmp;
+            T gain_basic = T(fac * tpow<T>(nrm2, -expo));
+            T ratio = nrm2 * nrm2 / err_prev;
+            T gain_gus = T(safety * (cur / dt_prev) * tpow<T>(ratio, -expo));
+            gain = gain_gus < gain_basic ? gain_gus : gain_basic;
+            gain = (accept && dt_prev > T(1e-16)) ? gain : gain_basic;
+            gain = clampv(gain, min_gain, max_gain);
+            gain = deadband(gain);
+            gain = accept ? gain : std::min(gain, safety);
+            dt_new_raw = cur * gain;
+            dt[0] = freeze ? cur : clampv(dt_new_raw, dt_min, dt_max);
+            history[0] = commit ? cur : history[0];
+            history[1] = commit ? nrm2 : history[1];
+            break;
+        }
+        default:
+            return ST_SUCCESS;
+        }
+        return (freeze || dt_new_raw > dt_min) ? ST_SUCCESS : ST_STEP_TOO_SMALL;
+    }
+};
+
+}  // namespace oracle
+
+#include "oracle_implicit.hpp"
+
+namespace oracle {
+
+// --------------------------------------------------------------------------
+// Summary metrics, outputhandling/summarymetrics/*.py
+// --------------------------------------------------------------------------
+static int metric_buffer_size(int id, int param) {
+    switch (id) {
+    case M_PEAKS: case M_NEGATIVE_PEAKS: return 3 + param;
+    case M_STD: case M_MEAN_STD_RMS: case M_MEAN_STD: case M_STD_RMS:
+    case M_DXDT_EXTREMA: case M_D2XDT2_MAX: case M_D2XDT2_MIN: return 3;
+    case M_EXTREMA: case M_DXDT_MAX: case M_DXDT_MIN: return 2;
+    case M_D2XDT2_EXTREMA: return 4;
+    default: return 1;
+    }
+}
+static int metric_output_size(int id, int param) {
+    switch (id) {
+    case M_PEAKS: case M_NEGATIVE_PEAKS: return param;
+    case M_MEAN_STD_RMS: return 3;
+    case M_MEAN_STD: case M_STD_RMS: case M_EXTREMA: case M_DXDT_EXTREMA:
+    case M_D2XDT2_EXTREMA: return 2;
+    default: return 1;
+    }
+}
+
+template <typename T>
+static void metric_update(int id, int param, T value, T* buf, int32_t idx) {
+    switch (id) {
+    case M_MEAN: buf[0] += value; break;
+    case M_MAX: if (value > buf[0]) buf[0] = value; break;
+    case M_MIN: if (value < buf[0]) buf[0] = value; break;
+    case M_RMS: {
+        T ss = buf[0];
+        if (idx == 0) ss = T(0);
+        ss += value * value;
+        buf[0] = ss;
+        break;
+    }
+    case M_PEAKS: case M_NEGATIVE_PEAKS: {
+        T prev = buf[0], prev_prev = buf[1];
+        int32_t counter = int32_t(buf[2]);
+        if (idx >= 2 && counter < param && prev_prev != T(0)) {
+            bool hit = (id == M_PEAKS) ? (prev > value && prev_prev < prev)
+                                       : (prev < value && prev_prev > prev);
+            if (hit) {
+                buf[3 + counter] = T(idx - 1);
+                buf[2] = T(int32_t(buf[2]) + 1);
+            }
+        }
+        buf[0] = value;
+        buf[1] = prev;
+        break;
+    }
+    case M_STD: case M_MEAN_STD: case M_MEAN_STD_RMS: case M_STD_RMS: {
+        if (idx == 0) { buf[0] = value; buf[1] = T(0); buf[2] = T(0); }
+        T sv = value - buf[0];
+        buf[1] += sv;
+        buf[2] += sv * sv;
+        break;
+    }
+    case M_MAX_MAGNITUDE: {
+        T a = std::fabs(value);
+        if (a > buf[0]) buf[0] = a;
+        break;
+    }
+    case M_EXTREMA:
+        if (value > buf[0]) buf[0] = value;
+        if (value < buf[1]) buf[1] = value;
+        break;
+    case M_DXDT_MAX: case M_DXDT_MIN: {
+        T du = value - buf[0];
+        bool upd = (id == M_DXDT_MAX ? du > buf[1] : du < buf[1]) && idx >= 1;
+        buf[1] = upd ? du : buf[1];
+        buf[0] = value;
+        break;
+    }
+    case M_DXDT_EXTREMA: {
+        T du = value - buf[0];
+        bool primed = idx >= 1;
+        bool umax = du > buf[1] && primed;
+        bool umin = du < buf[2] && primed;
+        buf[1] = umax ? du : buf[1];
+        buf[2] = umin ? du : buf[2];
+        buf[0] = value;
+        break;
+    }
+    case M_D2XDT2_MAX: case M_D2XDT2_MIN: {
+        T sd = value - T(2) * buf[0] + buf[1];
+        bool upd = (id == M_D2XDT2_MAX ? sd > buf[2] : sd < buf[2]) && idx >= 2;
+        buf[2] = upd ? sd : buf[2];
+        buf[1] = buf[0];
+        buf[0] = value;
+        break;
+    }
+    case M_D2XDT2_EXTREMA: {
+        T sd = value - T(2) * buf[0] + buf[1];
+        bool primed = idx >= 2;
+        bool umax = sd > buf[2] && primed;
+        bool umin = sd < buf[3] && primed;
+        buf[2] = umax ? sd : buf[2];
+        buf[3] = umin ? sd : buf[3];
+        buf[1] = buf[0];
+        buf[0] = value;
+        break;
+    }
+    default: break;
+    }
+}
+
+// Division of a precision-typed buffer by the int32 sample count follows
+// Numba's promotion (float / int32 -> float64) before the store narrows it.
+template <typename T>
+static void metric_save(int id, int param, T* buf, T* out, int32_t spp, T sample_every) {
+    switch (id) {
+    case M_MEAN:
+        out[0] = T(double(buf[0]) / double(spp));
+        buf[0] = T(0);
+        break;
+    case M_MAX: out[0] = buf[0]; buf[0] = T(-1.0e30); break;
+    case M_MIN: out[0] = buf[0]; buf[0] = T(1.0e30); break;
+    case M_RMS:
+        out[0] = T(std::sqrt(double(buf[0]) / double(spp)));
+        buf[0] = T(0);
+        break;
+    case M_PEAKS: case M_NEGATIVE_PEAKS:
+        for (int p = 0; p < param; ++p) { out[p] = buf[3 + p]; buf[3 + p] = T(0); }
+        buf[2] = T(0);
+        break;
+    case M_STD: {
+        double ms = double(buf[1]) / double(spp);
+        double mss = double(buf[2]) / double(spp);
+        double var = mss - ms * ms;
+        out[0] = T(std::sqrt(var));
+        buf[0] = T(double(buf[0]) + ms);
+        buf[1] = T(0); buf[2] = T(0);
+        break;
+    }
+    case M_MEAN_STD: case M_MEAN_STD_RMS: case M_STD_RMS: {
+        double shift = double(buf[0]);
+        double ms = double(buf[1]) / double(spp);
+        double mss = double(buf[2]) / double(spp);
+        double mean = shift + ms;
+        double var = mss - ms * ms;
+        double sd = std::sqrt(var);
+        double msq = mss + double(T(2) * buf[0]) * ms + double(buf[0] * buf[0]);
+        double rms = std::sqrt(msq);
+        if (id == M_MEAN_STD) { out[0] = T(mean); out[1] = T(sd); }
+        else if (id == M_MEAN_STD_RMS) { out[0] = T(mean); out[1] = T(sd); out[2] = T(rms); }
+        else { out[0] = T(sd); out[1] = T(rms); }
+        buf[0] = T(mean); buf[1] = T(0); buf[2] = T(0);
+        break;
+    }
+    case M_MAX_MAGNITUDE: out[0] = buf[0]; buf[0] = T(0); break;
+    case M_EXTREMA:
+        out[0] = buf[0]; out[1] = buf[1];
+        buf[0] = T(-1.0e30); buf[1] = T(1.0e30);
+        break;
+    case M_DXDT_MAX: out[0] = buf[1] / sample_every; buf[1] = T(-1.0e30); break;
+    case M_DXDT_MIN: out[0] = buf[1] / sample_every; buf[1] = T(1.0e30); break;
+    case M_DXDT_EXTREMA:
+        out[0] = buf[1] / sample_every; out[1] = buf[2] / sample_every;
+        buf[1] = T(-1.0e30); buf[2] = T(1.0e30);
+        break;
+    case M_D2XDT2_MAX: case M_D2XDT2_MIN: {
+        T sq = sample_every * sample_every;
+        out[0] = buf[2] / sq;
+        buf[2] = T(id == M_D2XDT2_MAX ? -1.0e30 : 1.0e30);
+        break;
+    }
+    case M_D2XDT2_EXTREMA: {
+        T sq = sample_every * sample_every;
+        out[0] = buf[2] / sq; out[1] = buf[3] / sq;
+        buf[2] = T(-1.0e30); buf[3] = T(1.0e30);
+        break;
+    }
+    default: break;
+    }
+}
+
+// --------------------------------------------------------------------------
+// Output writers
+// --------------------------------------------------------------------------
+struct Outputs {
+    double* state;        // (rows, n_saved_states + save_time, N)
+    double* obs;          // (rows, n_saved_obs, N)
+    double* state_summ;   // (srows, n_summ_states * out_h, N)
+    double* obs_summ;     // (srows, n_summ_obs * out_h, N)
+    int32_t* counters;    // (rows, 4, N)
+    int32_t* status;      // (N,)
+    int64_t N;
+};
+
+// --------------------------------------------------------------------------
+// The per-system loop, ode_loop.py:562-984
+// --------------------------------------------------------------------------
+template <typename T, typename Stepper>
+static int32_t run_loop(const OracleConfig* cfg, Problem<T>& P, Stepper& stepper,
+                        const T* init, const T* par, const Outputs& out, int64_t run) {
+    const int n = cfg->n_states;
+    const int n_obs = cfg->n_obs;
+    const int n_drv = cfg->n_drivers;
+    const int64_t N = out.N;
+    const bool adaptive = cfg->controller != CTRL_FIXED;
+    const bool summarise = cfg->summarise_state || cfg->summarise_obs;
+    const int n_err = cfg->has_error ? n : 0;
+
+    ControllerState<T> ctrl;
+    ctrl.init(cfg, n);
+
+    T duration = T(cfg->duration), settling = T(cfg->warmup), t0 = T(cfg->t0);
+    T save_stop = T(cfg->save_stop), summary_stop = T(cfg->summary_stop);
+    T save_every = T(cfg->save_every);
+    T sample_every = T(cfg->sample_summaries_every);
+    const int32_t spp = cfg->samples_per_summary;
+
+    double t = double(t0);
+    T t_prec = T(t);
+    T t_end = T(settling + t0 + duration);
+
+    T state[MAXN], proposal[MAXN], params[MAXN], drivers[16], prop_drivers[16];
+    T obs[MAXN], prop_obs[MAXN], error[MAXN];
+    for (int i = 0; i < MAXN; ++i) { state[i] = proposal[i] = params[i] = obs[i] = prop_obs[i] = error[i] = T(0); }
+    for (int i = 0; i < 16; ++i) drivers[i] = prop_drivers[i] = T(0);
+    const int H = cfg->summ_buf_height;
+    std::vector<T> ssum(size_t(std::max(1, cfg->n_summ_states * H)), T(0));
+    std::vector<T> osum(size_t(std::max(1, cfg->n_summ_obs * H)), T(0));
+    int32_t counters_since_save[4] = {0, 0, 0, 0};
+    int32_t proposed_counters[2] = {0, 0};
+
+    bool first_step = true;
+    bool prev_accepted = true;
+    int32_t stagnant = 0;
+    int32_t save_idx = 0, summary_idx = 0, update_idx = 0;
+    T next_save = T(settling + t0);
+    T next_update = T(settling + t0);
+
+    for (int i = 0; i < n; ++i) state[i] = init[i];
+    for (int i = 0; i < cfg->n_params; ++i) params[i] = par[i];
+    if (n_drv > 0) P.drivers(t_prec, drivers);
+    if (n_obs > 0) P.observables(state, params, drivers, obs, t_prec);
+
+    const int out_h = cfg->summ_out_height;
+    auto save_state_fn = [&](int32_t row) {
+        if (row >= cfg->n_out_rows) return;  // guarded store
+        const int width = cfg->n_saved_states + (cfg->save_time ? 1 : 0);
+        if (cfg->save_state) {
+            for (int k = 0; k < cfg->n_saved_states; ++k)
+                out.state[(int64_t(row) * width + k) * N + run] = double(state[cfg->saved_state_idx[k]]);
+        }
+        if (cfg->save_time)
+            out.state[(int64_t(row) * width + cfg->n_saved_states) * N + run] = double(t_prec);
+        if (cfg->save_obs) {
+            for (int k = 0; k < cfg->n_saved_obs; ++k)
+                out.obs[(int64_t(row) * cfg->n_saved_obs + k) * N + run] = double(obs[cfg->saved_obs_idx[k]]);
+        }
+        if (cfg->save_counters) {
+            for (int k = 0; k < 4; ++k)
+                out.counters[(int64_t(row) * 4 + k) * N + run] = counters_since_save[k];
+        }
+    };
+    auto chain_update = [&](T value, T* buf) {
+        int off = 0;
+        for (int m = 0; m < cfg->n_metrics; ++m) {
+            metric_update<T>(cfg->metric_ids[m], cfg->metric_params[m], value, buf + off, update_idx);
+            off += metric_buffer_size(cfg->metric_ids[m], cfg->metric_params[m]);
+        }
+    };
+    auto chain_save = [&](T* buf, T* outv) {
+        int off = 0, ooff = 0;
+        for (int m = 0; m < cfg->n_metrics; ++m) {
+            int id = cfg->metric_ids[m], prm = cfg->metric_params[m];
+            metric_save<T>(id, prm, buf + off, outv + ooff, spp, sample_every);
+            off += metric_buffer_size(id, prm);
+            ooff += metric_output_size(id, prm);
+        }
+    };
+    auto save_summaries_fn = [&](int32_t row) {
+        T tmp[256];
+        if (cfg->summarise_state) {
+            for (int v = 0; v < cfg->n_summ_states; ++v) {
+                chain_save(&ssum[size_t(v) * H], tmp);
+                if (row < cfg->n_summary_rows)
+                    for (int k = 0; k < out_h; ++k)
+                        out.state_summ[(int64_t(row) * (cfg->n_summ_states * out_h) + v * out_h + k) * N + run] = double(tmp[k]);
+            }
+        }
+        if (cfg->summarise_obs) {
+            for (int v = 0; v < cfg->n_summ_obs; ++v) {
+                chain_save(&osum[size_t(v) * H], tmp);
+                if (row < cfg->n_summary_rows)
+                    for (int k = 0; k < out_h; ++k)
+                        out.obs_summ[(int64_t(row) * (cfg->n_summ_obs * out_h) + v * out_h + k) * N + run] = double(tmp[k]);
+            }
+        }
+    };
+    auto update_summaries_fn = [&]() {
+        if (cfg->summarise_state)
+            for (int v = 0; v < cfg->n_summ_states; ++v)
+                chain_update(state[cfg->summ_state_idx[v]], &ssum[size_t(v) * H]);
+        if (cfg->summarise_obs)
+            for (int v = 0; v < cfg->n_summ_obs; ++v)
+                chain_update(obs[cfg->summ_obs_idx[v]], &osum[size_t(v) * H]);
+    };
+
+    if (settling == T(0)) {
+        if (cfg->save_regularly) next_save = T(next_save + save_every);
+        if (cfg->summarise_regularly) next_update = T(sample_every + next_update);
+        save_state_fn(save_idx);
+        save_idx += 1;
+        if (summarise) save_summaries_fn(summary_idx);  // resets buffers only
+    }
+
+    int32_t status = ST_SUCCESS;
+    int32_t iteration_status = 0;
+    T dt[1] = {T(cfg->dt)};
+    T dt_raw = dt[0];
+    int32_t accept_step = 0;
+    bool irrecoverable = false;
+    bool at_end = false, save_finished = false, summary_finished = false;
+
+    // Hard cap on iterations so a broken configuration cannot hang a test.
+    const int64_t max_iterations = int64_t(1) << 40;
+    for (int64_t iter = 0; iter < max_iterations; ++iter) {
+        T end_of_step = t_prec + dt_raw;
+        bool finished;
+        if (cfg->save_regularly || cfg->summarise_regularly) {
+            finished = true;
+            if (cfg->save_regularly) {
+                save_finished = next_save > save_stop;
+                finished = finished && save_finished;
+            }
+            if (cfg->summarise_regularly) {
+                summary_finished = next_update > summary_stop;
+                finished = finished && summary_finished;
+            }
+        } else {
+            finished = end_of_step >= t_end;
+        }
+        if (cfg->save_last) {
+            at_end = (t_prec < t_end) && finished;
+            finished = finished && !at_end;
+        }
+        finished = finished || irrecoverable;
+        if (finished) break;
+
+        bool do_save = cfg->save_regularly ? ((end_of_step >= next_save) && !save_finished) : false;
+        bool do_update = cfg->summarise_regularly ? ((end_of_step >= next_update) && !summary_finished) : false;
+        if (cfg->save_last) do_save = do_save || at_end;
+
+        T dt_eff = dt_raw;
+        bool truncated = false;
+        if (do_save || do_update) {
+            T next_event = t_end;
+            if (do_save && cfg->save_regularly) next_event = std::min(next_event, next_save);
+            if (do_update && cfg->summarise_regularly) next_event = std::min(next_event, next_update);
+            T gap = T(next_event - t_prec);
+            dt_eff = gap > T(0) ? gap : dt_raw;
+            truncated = dt_eff != dt_raw;
+        }
+
+        int32_t step_status = stepper.step(state, proposal, params, drivers, prop_drivers,
+                                           obs, prop_obs, error, dt_eff, t_prec,
+                                           first_step, prev_accepted, proposed_counters);
+        first_step = false;
+        int32_t niters = proposed_counters[0];
+        iteration_status |= step_status;
+        bool step_failed = step_status != 0;
+        irrecoverable = irrecoverable || (!adaptive && step_failed);
+        for (int i = 0; i < n_err; ++i) error[i] = step_failed ? T(1e16) : error[i];
+
+        bool accept;
+        if (adaptive) {
+            int32_t cs = ctrl.control(dt, proposal, state, error, niters, truncated, &accept_step);
+            accept = (accept_step != 0) && !step_failed;
+            iteration_status |= cs;
+            irrecoverable = irrecoverable || ((cs & ST_STEP_TOO_SMALL) != 0);
+        } else {
+            accept = !step_failed;
+        }
+        dt_raw = dt[0];
+
+        if (cfg->save_counters) {
+            counters_since_save[0] += proposed_counters[0];
+            counters_since_save[1] += proposed_counters[1];
+            counters_since_save[2] += 1;
+            if (!accept) counters_since_save[3] += 1;
+        }
+
+        double t_proposal = t + double(dt_eff);
+        if (t_proposal == t) stagnant += 1; else stagnant = 0;
+        bool is_stagnant = stagnant >= 2;
+        if (is_stagnant) iteration_status |= ST_STAGNATION;
+        irrecoverable = irrecoverable || is_stagnant;
+        if (irrecoverable) status |= iteration_status;
+        if (accept) iteration_status = 0;
+
+        if (accept) t = t_proposal;
+        t_prec = T(t);
+        if (accept) {
+            for (int i = 0; i < n; ++i) state[i] = proposal[i];
+            for (int i = 0; i < n_drv; ++i) drivers[i] = prop_drivers[i];
+            for (int i = 0; i < n_obs; ++i) obs[i] = prop_obs[i];
+        }
+        prev_accepted = accept;
+
+        do_save = do_save && accept;
+        do_update = do_update && accept;
+        if (do_save) {
+            if (cfg->save_regularly) next_save += save_every;
+            save_state_fn(save_idx);
+            save_idx += 1;
+            if (cfg->save_counters)
+                for (int k = 0; k < 4; ++k) counters_since_save[k] = 0;
+        }
+        if (do_update) {
+            if (cfg->summarise_regularly) next_update += sample_every;
+            if (summarise) {
+                update_summaries_fn();
+                update_idx += 1;
+                if (spp > 0 && update_idx % spp == 0) {
+                    save_summaries_fn(summary_idx);
+                    summary_idx += 1;
+                }
+            }
+        }
+    }
+    return status;
+}
+
+template <typename T, typename Stepper>
+static int solve_typed(const OracleConfig* cfg, int64_t n_runs, const double* inits,
+                       const double* params, Outputs out, int64_t run_begin, int64_t run_end) {
+    Problem<T> P;
+    P.cfg = cfg;
+    P.n = cfg->n_states;
+    P.sys.id = cfg->system_id;
+    P.sys.n = cfg->n_states;
+    P.sys.n_params = cfg->n_params;
+    P.sys.n_obs = cfg->n_obs;
+    P.sys.n_drivers = cfg->n_drivers;
+    P.sys.constants = cfg->constants;
+    P.sys.n_constants = cfg->n_constants;
+    P.sys.rhs_cb = cfg->rhs_cb;
+    P.sys.obs_cb = cfg->obs_cb;
+    P.sys.jvp_cb = cfg->jvp_cb;
+    P.drv.init(cfg);
+    for (int64_t r = run_begin; r < run_end; ++r) {
+        T init[MAXN], par[MAXN];
+        for (int i = 0; i < cfg->n_states; ++i) init[i] = T(inits[int64_t(i) * n_runs + r]);
+        for (int i = 0; i < cfg->n_params; ++i) par[i] = T(params[int64_t(i) * n_runs + r]);
+        Stepper st;
+        st.init(&P);
+        out.status[r] = run_loop<T, Stepper>(cfg, P, st, init, par, out, r);
+    }
+    return 0;
+}
+
+template <typename T>
+static int dispatch_family(const OracleConfig* cfg, int64_t n_runs, const double* inits,
+                           const double* params, Outputs out, int64_t b, int64_t e) {
+    switch (cfg->family) {
+    case FAM_ERK: return solve_typed<T, ERKStepper<T>>(cfg, n_runs, inits, params, out, b, e);
+    case FAM_DIRK: return solve_typed<T, DIRKStepper<T>>(cfg, n_runs, inits, params, out, b, e);
+    case FAM_FIRK: return solve_typed<T, FIRKStepper<T>>(cfg, n_runs, inits, params, out, b, e);
+    case FAM_ROSENBROCK: return solve_typed<T, RosenbrockStepper<T>>(cfg, n_runs, inits, params, out, b, e);
+    default: return -2;
+    }
+}
+
+}  // namespace oracle
+
+extern "C" {
+
+// Integrate runs [run_begin, run_end) of a batch of n_runs systems.
+// Inputs are (n_states, n_runs) and (n_params, n_runs) row-major float64;
+// outputs use the (time, variable, run) layout. Returns 0 on success.
+int oracle_solve(const oracle::OracleConfig* cfg, int64_t n_runs,
+                 const double* inits, const double* params,
+                 double* state_out, double* obs_out, double* state_summ_out,
+                 double* obs_summ_out, int32_t* counters_out, int32_t* status_out,
+                 int64_t run_begin, int64_t run_end) {
+    if (cfg->n_states > oracle::MAXN || cfg->n_params > oracle::MAXN ||
+        cfg->n_obs > oracle::MAXN || cfg->n_drivers > 16 || cfg->stages > oracle::MAXS)
+        return -1;
+    oracle::Outputs out{state_out, obs_out, state_summ_out, obs_summ_out,
+                        counters_out, status_out, n_runs};
+    if (run_end > n_runs) run_end = n_runs;
+    if (cfg->precision_bits == 32)
+        return oracle::dispatch_family<float>(cfg, n_runs, inits, params, out, run_begin, run_end);
+    return oracle::dispatch_family<double>(cfg, n_runs, inits, params, out, run_begin, run_end);
+}
+
+int oracle_abi_version(void) { return 1; }
+
+}  // extern "C"

@@ -1,0 +1,320 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use
+this module, always as the checker. It builds ``liboracle.so`` from
+oracle/cubie_oracle.cpp (``make -C oracle``) and runs the restated reference
+loop on the CPU for a configuration described by a cubie_amd
+``BatchSolverKernel`` (raw settings only: the oracle derives its own gain
+exponents, casts and tableau properties from the raw values).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+from typing import Dict, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+SYSTEM_IDS = {"lorenz": 0, "lorenz_julia": 1, "robertson": 2, "hodgkin_huxley": 3,
+              "van_der_pol": 4, "linear3": 5, "callback": 6, "decay": 7}
+FAMILY_IDS = {"erk": 0, "dirk": 1, "firk": 2, "rosenbrock": 3}
+LINEAR_IDS = {"minimal_residual": 0, "steepest_descent": 1, "bicgstab": 2}
+
+RHS_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double), ctypes.c_double)
+OBS_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.c_double)
+JVP_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double), ctypes.c_double)
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+
+
+class OracleConfig(ctypes.Structure):
+    _fields_ = [
+        ("precision_bits", ctypes.c_int32), ("system_id", ctypes.c_int32),
+        ("n_states", ctypes.c_int32), ("n_params", ctypes.c_int32),
+        ("n_obs", ctypes.c_int32), ("n_drivers", ctypes.c_int32),
+        ("constants", _D), ("n_constants", ctypes.c_int32),
+        ("family", ctypes.c_int32), ("stages", ctypes.c_int32), ("order", ctypes.c_int32),
+        ("has_error", ctypes.c_int32),
+        ("a", _D), ("b", _D), ("b_hat", _D), ("c", _D),
+        ("rb_C", _D), ("rb_gamma_stages", _D), ("rb_gamma", ctypes.c_double),
+        ("controller", ctypes.c_int32),
+        ("dt", ctypes.c_double), ("dt_min", ctypes.c_double), ("dt_max", ctypes.c_double),
+        ("kp", ctypes.c_double), ("ki", ctypes.c_double), ("kd", ctypes.c_double),
+        ("min_gain", ctypes.c_double), ("max_gain", ctypes.c_double),
+        ("safety", ctypes.c_double), ("deadband_min", ctypes.c_double),
+        ("deadband_max", ctypes.c_double), ("newton_target_iters", ctypes.c_int32),
+        ("atol", _D), ("rtol", _D),
+        ("save_every", ctypes.c_double), ("summarise_every", ctypes.c_double),
+        ("sample_summaries_every", ctypes.c_double),
+        ("save_last", ctypes.c_int32), ("save_regularly", ctypes.c_int32),
+        ("summarise_regularly", ctypes.c_int32), ("samples_per_summary", ctypes.c_int32),
+        ("save_state", ctypes.c_int32), ("save_obs", ctypes.c_int32),
+        ("save_time", ctypes.c_int32), ("save_counters", ctypes.c_int32),
+        ("summarise_state", ctypes.c_int32), ("summarise_obs", ctypes.c_int32),
+        ("n_saved_states", ctypes.c_int32), ("n_saved_obs", ctypes.c_int32),
+        ("n_summ_states", ctypes.c_int32), ("n_summ_obs", ctypes.c_int32),
+        ("saved_state_idx", _I), ("saved_obs_idx", _I), ("summ_state_idx", _I),
+        ("summ_obs_idx", _I),
+        ("n_metrics", ctypes.c_int32), ("metric_ids", _I), ("metric_params", _I),
+        ("summ_buf_height", ctypes.c_int32), ("summ_out_height", ctypes.c_int32),
+        ("n_out_rows", ctypes.c_int32), ("n_summary_rows", ctypes.c_int32),
+        ("drv_order", ctypes.c_int32), ("drv_segments", ctypes.c_int32),
+        ("drv_wrap", ctypes.c_int32), ("drv_pad_clamped", ctypes.c_int32),
+        ("drv_t0", ctypes.c_double), ("drv_dt", ctypes.c_double), ("drv_coeffs", _D),
+        ("duration", ctypes.c_double), ("warmup", ctypes.c_double), ("t0", ctypes.c_double),
+        ("save_stop", ctypes.c_double), ("summary_stop", ctypes.c_double),
+        ("newton_max_iters", ctypes.c_int32), ("krylov_max_iters", ctypes.c_int32),
+        ("linear_solver", ctypes.c_int32), ("preconditioner_order", ctypes.c_int32),
+        ("newton_atol_s", ctypes.c_double), ("newton_rtol_s", ctypes.c_double),
+        ("krylov_atol_s", ctypes.c_double), ("krylov_rtol_s", ctypes.c_double),
+        ("krylov_residual_reduction", ctypes.c_double), ("krylov_floor", ctypes.c_double),
+        ("rhs_cb", RHS_CB), ("obs_cb", OBS_CB), ("jvp_cb", JVP_CB),
+    ]
+
+
+def build(force: bool = False) -> str:
+    src = os.path.join(HERE, "cubie_oracle.cpp")
+    deps = [src, os.path.join(HERE, "systems.hpp"), os.path.join(HERE, "oracle_implicit.hpp")]
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(d) for d in deps):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.oracle_solve.argtypes = [ctypes.POINTER(OracleConfig), ctypes.c_int64, _D, _D, _D, _D,
+                                   _D, _D, _I, _I, ctypes.c_int64, ctypes.c_int64]
+        L.oracle_solve.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+class OracleRun:
+    """Holds the config and every array it points into (keeps them alive)."""
+
+    def __init__(self, kernel, system_id: str, duration: float, warmup: float = 0.0,
+                 t0: float = 0.0, constants=None, callbacks=None):
+        from cubie_amd.outputhandling.summary_metrics import summary_metrics
+        k = kernel
+        sysm = k.system
+        prec = k.precision
+        alg = k.algorithm
+        tab = alg.tableau
+        ctrl = k.controller
+        oc = k.output_config
+        timing = k.timing.with_duration(duration)
+        cfg = OracleConfig()
+        keep = []
+
+        def arr(x, dt=np.float64):
+            a = np.ascontiguousarray(np.asarray(x, dtype=dt).reshape(-1))
+            if a.size == 0:
+                a = np.zeros(1, dtype=dt)
+            keep.append(a)
+            return a
+
+        cfg.precision_bits = 32 if prec is np.float32 else 64
+        cfg.system_id = SYSTEM_IDS[system_id]
+        cfg.n_states = sysm.num_states
+        cfg.n_params = sysm.num_parameters
+        cfg.n_obs = sysm.num_observables
+        cfg.n_drivers = sysm.num_drivers
+        consts = constants if constants is not None else [sysm.constants[c] for c in sysm.constant_names]
+        cfg.constants = _dp(arr(consts))
+        cfg.n_constants = len(consts)
+        cfg.family = FAMILY_IDS[alg.family]
+        s = tab.stage_count
+        cfg.stages = s
+        cfg.order = tab.order
+        cfg.has_error = int(alg.is_adaptive)
+        cfg.a = _dp(arr(np.array(tab.a)))
+        cfg.b = _dp(arr(tab.b))
+        cfg.b_hat = _dp(arr(tab.b_hat if tab.b_hat is not None else np.zeros(s)))
+        cfg.c = _dp(arr(tab.c))
+        if alg.family == "rosenbrock":
+            cfg.rb_C = _dp(arr(np.array(tab.C)))
+            cfg.rb_gamma_stages = _dp(arr(tab.gamma_stages))
+            cfg.rb_gamma = tab.gamma
+        else:
+            cfg.rb_C = _dp(arr(np.zeros(s * s)))
+            cfg.rb_gamma_stages = _dp(arr(np.zeros(s)))
+            cfg.rb_gamma = 0.0
+        cfg.controller = ctrl.id
+        cfg.dt, cfg.dt_min, cfg.dt_max = ctrl.dt, ctrl.dt_min, ctrl.dt_max
+        cfg.kp, cfg.ki, cfg.kd = ctrl.kp, ctrl.ki, ctrl.kd
+        cfg.min_gain, cfg.max_gain, cfg.safety = ctrl.min_gain, ctrl.max_gain, ctrl.safety
+        cfg.deadband_min, cfg.deadband_max = ctrl.deadband_min, ctrl.deadband_max
+        cfg.newton_target_iters = ctrl.newton_target_iters
+        cfg.atol = _dp(arr(ctrl.atol))
+        cfg.rtol = _dp(arr(ctrl.rtol))
+        cfg.save_every = timing.save_every or 0.0
+        cfg.summarise_every = timing.summarise_every or 0.0
+        cfg.sample_summaries_every = timing.sample_summaries_every or 0.0
+        cfg.save_last = int(timing.save_last)
+        cfg.save_regularly = int(timing.save_regularly)
+        cfg.summarise_regularly = int(timing.summarise_regularly)
+        cfg.samples_per_summary = timing.samples_per_summary if oc.save_summaries else 0
+        cfg.save_state = int(oc.save_state)
+        cfg.save_obs = int(oc.save_observables)
+        cfg.save_time = int(oc.save_time)
+        cfg.save_counters = int(oc.save_counters)
+        cfg.summarise_state = int(oc.summarise_state)
+        cfg.summarise_obs = int(oc.summarise_observables)
+        ssi = oc.saved_state_indices if oc.save_state else []
+        soi = oc.saved_observable_indices if oc.save_observables else []
+        smi = oc.summarised_state_indices if oc.summarise_state else []
+        smo = oc.summarised_observable_indices if oc.summarise_observables else []
+        cfg.n_saved_states, cfg.n_saved_obs = len(ssi), len(soi)
+        cfg.n_summ_states, cfg.n_summ_obs = len(smi), len(smo)
+        cfg.saved_state_idx = _ip(arr(ssi, np.int32))
+        cfg.saved_obs_idx = _ip(arr(soi, np.int32))
+        cfg.summ_state_idx = _ip(arr(smi, np.int32))
+        cfg.summ_obs_idx = _ip(arr(smo, np.int32))
+        mets = oc.metrics
+        cfg.n_metrics = len(mets)
+        cfg.metric_ids = _ip(arr([summary_metrics[m].metric_id for m, _ in mets], np.int32))
+        cfg.metric_params = _ip(arr([p for _, p in mets], np.int32))
+        cfg.summ_buf_height = oc.summaries_buffer_height_per_var
+        cfg.summ_out_height = oc.summaries_output_height_per_var
+        dur_p = float(prec(duration))
+        warm_p = float(prec(warmup))
+        t0_p = float(prec(t0))
+        shapes = k.output_shapes(timing, dur_p, 1)
+        cfg.n_out_rows = shapes["state"][0] or shapes["observables"][0] or shapes["iteration_counters"][0]
+        cfg.n_summary_rows = max(shapes["state_summaries"][0], shapes["observable_summaries"][0])
+        d = k.driver_interpolator
+        if d is not None and sysm.num_drivers:
+            cfg.drv_order = d.order
+            cfg.drv_segments = d.num_segments
+            cfg.drv_wrap = int(d.wrap)
+            cfg.drv_pad_clamped = int(d.pad_clamped)
+            cfg.drv_t0 = d.t0
+            cfg.drv_dt = d.driver_sample_period
+            cfg.drv_coeffs = _dp(arr(np.asarray(d.coefficients, dtype=np.float64)))
+        else:
+            cfg.drv_segments = 1
+            cfg.drv_dt = 1.0
+            cfg.drv_coeffs = _dp(arr([0.0]))
+        cfg.duration, cfg.warmup, cfg.t0 = dur_p, warm_p, t0_p
+        cfg.save_stop = float(prec(timing.save_stop_time(dur_p, warm_p, t0_p)))
+        cfg.summary_stop = float(prec(timing.summary_stop_time(dur_p, warm_p, t0_p)))
+        im = k.implicit
+        cfg.newton_max_iters = im.newton_max_iters
+        cfg.krylov_max_iters = im.krylov_max_iters
+        cfg.linear_solver = LINEAR_IDS[im.linear_correction_type]
+        cfg.preconditioner_order = im.preconditioner_order
+        cfg.newton_atol_s, cfg.newton_rtol_s = im.newton_atol, im.newton_rtol
+        cfg.krylov_atol_s, cfg.krylov_rtol_s = im.krylov_atol, im.krylov_rtol
+        cfg.krylov_residual_reduction = (im.krylov_residual_reduction
+                                         if im.krylov_residual_reduction is not None
+                                         else float(np.finfo(prec).eps))
+        cfg.krylov_floor = (im.krylov_residual_floor if im.krylov_residual_floor is not None
+                            else float(np.sqrt(np.finfo(prec).eps)))
+        if callbacks is not None:
+            cfg.rhs_cb, cfg.obs_cb, cfg.jvp_cb = callbacks
+            keep.append(callbacks)
+        else:
+            cfg.rhs_cb = RHS_CB()
+            cfg.obs_cb = OBS_CB()
+            cfg.jvp_cb = JVP_CB()
+        self.cfg = cfg
+        self.keep = keep
+        self.kernel = k
+        self.timing = timing
+        self.dur_p = dur_p
+
+    def solve(self, inits: np.ndarray, params: np.ndarray, threads: int = 1,
+              runs: Optional[range] = None) -> Dict[str, np.ndarray]:
+        inits = np.ascontiguousarray(inits, dtype=np.float64)
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        N = inits.shape[1]
+        shapes = self.kernel.output_shapes(self.timing, self.dur_p, N)
+        out = {}
+        for key, shp in shapes.items():
+            dt = np.int32 if key == "iteration_counters" else np.float64
+            out[key] = np.zeros(max(int(np.prod(shp)), 1), dtype=dt)
+        status = np.zeros(N, dtype=np.int32)
+        L = lib()
+        pr = params if params.size else np.zeros((1, N))
+        begin, end = (0, N) if runs is None else (runs.start, runs.stop)
+
+        def work(b, e):
+            return L.oracle_solve(ctypes.byref(self.cfg), N, _dp(inits), _dp(pr),
+                                  _dp(out["state"]), _dp(out["observables"]),
+                                  _dp(out["state_summaries"]), _dp(out["observable_summaries"]),
+                                  _ip(out["iteration_counters"]), _ip(status), b, e)
+
+        if threads <= 1 or self.cfg.system_id == SYSTEM_IDS["callback"]:
+            rc = work(begin, end)
+            if rc != 0:
+                raise RuntimeError(f"oracle_solve failed ({rc})")
+        else:
+            edges = np.linspace(begin, end, threads + 1).astype(int)
+            with ThreadPoolExecutor(threads) as ex:
+                rcs = list(ex.map(lambda ij: work(*ij), zip(edges[:-1], edges[1:])))
+            if any(rcs):
+                raise RuntimeError(f"oracle_solve failed {rcs}")
+        res = {k: (v[: int(np.prod(shapes[k]))].reshape(shapes[k]) if np.prod(shapes[k]) else
+                   np.zeros(shapes[k], dtype=v.dtype)) for k, v in out.items()}
+        res["status_codes"] = status
+        return res
+
+
+def callbacks_for(system):
+    """ctypes callbacks evaluating a SymbolicODE with NumPy (slow; tiny cases)."""
+    rhs, obs, jvp = system.numpy_rhs()
+    n, p_, nd, no = system.num_states, system.num_parameters, system.num_drivers, system.num_observables
+
+    def _r(y, p, d, o, dy, t):
+        yy = [y[i] for i in range(n)]
+        pp = [p[i] for i in range(p_)]
+        dd = [d[i] for i in range(nd)]
+        vals = rhs(yy, pp, dd, t)
+        for i in range(n):
+            dy[i] = float(vals[i])
+        if no:
+            ov = obs(yy, pp, dd, t)
+            for i in range(no):
+                o[i] = float(ov[i])
+
+    def _o(y, p, d, o, t):
+        if no:
+            ov = obs([y[i] for i in range(n)], [p[i] for i in range(p_)], [d[i] for i in range(nd)], t)
+            for i in range(no):
+                o[i] = float(ov[i])
+
+    def _j(y, p, d, v, jv, t):
+        vals = jvp([y[i] for i in range(n)], [p[i] for i in range(p_)], [d[i] for i in range(nd)], t,
+                   [v[i] for i in range(n)])
+        for i in range(n):
+            jv[i] = float(vals[i])
+
+    return RHS_CB(_r), OBS_CB(_o), JVP_CB(_j)

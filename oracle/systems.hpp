@@ -1,0 +1,238 @@
+// Hand-written right-hand sides for the oracle (test infrastructure only).
+//
+// These systems are written out by hand, independently of the package's
+// SymPy -> HIP code generator, so that a mistake in generated code cannot
+// also hide in the checker. Each system provides f(t, y), its observables
+// and an analytic Jacobian-vector product J(y) v.
+//
+// System equations follow the reference's test fixtures
+// (tests/system_fixtures.py:47-83 three-state linear, :513-537 Lorenz-Julia)
+// and the textbook models named by the benchmark configurations
+// (Lorenz 1963, Robertson 1966, Hodgkin-Huxley 1952, van der Pol).
+#pragma once
+#include <cmath>
+
+namespace oracle {
+
+enum SystemId {
+    SYS_LORENZ = 0,        // params sigma, rho, beta
+    SYS_LORENZ_JULIA = 1,  // param rho; constants sigma, beta
+    SYS_ROBERTSON = 2,     // params k1, k2, k3
+    SYS_HODGKIN_HUXLEY = 3,// params gNa, gK, gL; driver I_ext
+    SYS_VAN_DER_POL = 4,   // param mu
+    SYS_LINEAR3 = 5,       // params p0..p2, constants c0..c2, driver d0, obs o0..o2
+    SYS_CALLBACK = 6,      // Python callbacks
+    SYS_DECAY = 7,         // dy_i = -k_i y_i, params k_i (closed form)
+};
+
+// Callback signatures used by SYS_CALLBACK (always double precision).
+typedef void (*rhs_cb_t)(const double* y, const double* p, const double* d,
+                         double* obs, double* dy, double t);
+typedef void (*obs_cb_t)(const double* y, const double* p, const double* d,
+                         double* obs, double t);
+typedef void (*jvp_cb_t)(const double* y, const double* p, const double* d,
+                         const double* v, double* jv, double t);
+
+struct SystemDesc {
+    int id;
+    int n, n_params, n_obs, n_drivers;
+    const double* constants;
+    int n_constants;
+    rhs_cb_t rhs_cb;
+    obs_cb_t obs_cb;
+    jvp_cb_t jvp_cb;
+};
+
+// Hodgkin-Huxley constants (squid axon, resting potential -65 mV convention).
+static const double HH_ENA = 50.0;
+static const double HH_EK = -77.0;
+static const double HH_EL = -54.387;
+static const double HH_CM = 1.0;
+
+template <typename T>
+inline void hh_rates(T V, T& am, T& bm, T& ah, T& bh, T& an, T& bn) {
+    am = T(0.1) * (V + T(40.0)) / (T(1.0) - std::exp(-(V + T(40.0)) / T(10.0)));
+    bm = T(4.0) * std::exp(-(V + T(65.0)) / T(18.0));
+    ah = T(0.07) * std::exp(-(V + T(65.0)) / T(20.0));
+    bh = T(1.0) / (T(1.0) + std::exp(-(V + T(35.0)) / T(10.0)));
+    an = T(0.01) * (V + T(55.0)) / (T(1.0) - std::exp(-(V + T(55.0)) / T(10.0)));
+    bn = T(0.125) * std::exp(-(V + T(65.0)) / T(80.0));
+}
+
+template <typename T>
+void sys_observables(const SystemDesc& s, const T* y, const T* p, const T* d,
+                     T* obs, T t) {
+    switch (s.id) {
+    case SYS_LINEAR3: {
+        const T c0 = T(s.constants[0]), c1 = T(s.constants[1]), c2 = T(s.constants[2]);
+        T dx0 = -y[0], dx1 = -y[1] / T(2), dx2 = -y[2] / T(3);
+        obs[0] = dx0 * p[0] + c0 + d[0];
+        obs[1] = dx1 * p[1] + c1 + d[0];
+        obs[2] = dx2 * p[2] + c2 + d[0];
+        break;
+    }
+    case SYS_CALLBACK: {
+        if (s.n_obs == 0) break;
+        double yd[64], pd[64], dd[16], od[64];
+        for (int i = 0; i < s.n; ++i) yd[i] = double(y[i]);
+        for (int i = 0; i < s.n_params; ++i) pd[i] = double(p[i]);
+        for (int i = 0; i < s.n_drivers; ++i) dd[i] = double(d[i]);
+        s.obs_cb(yd, pd, dd, od, double(t));
+        for (int i = 0; i < s.n_obs; ++i) obs[i] = T(od[i]);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+template <typename T>
+void sys_rhs(const SystemDesc& s, const T* y, const T* p, const T* d, T* obs,
+             T* dy, T t) {
+    switch (s.id) {
+    case SYS_LORENZ:
+        dy[0] = p[0] * (y[1] - y[0]);
+        dy[1] = y[0] * (p[1] - y[2]) - y[1];
+        dy[2] = y[0] * y[1] - p[2] * y[2];
+        break;
+    case SYS_LORENZ_JULIA: {
+        const T sigma = T(s.constants[0]), beta = T(s.constants[1]);
+        dy[0] = sigma * (y[1] - y[0]);
+        dy[1] = y[0] * (p[0] - y[2]) - y[1];
+        dy[2] = y[0] * y[1] - beta * y[2];
+        break;
+    }
+    case SYS_ROBERTSON: {
+        const T r1 = p[0] * y[0];
+        const T r2 = p[1] * y[1] * y[1];
+        const T r3 = p[2] * y[1] * y[2];
+        dy[0] = -r1 + r3;
+        dy[1] = r1 - r2 - r3;
+        dy[2] = r2;
+        break;
+    }
+    case SYS_HODGKIN_HUXLEY: {
+        T am, bm, ah, bh, an, bn;
+        const T V = y[0], m = y[1], h = y[2], n = y[3];
+        hh_rates<T>(V, am, bm, ah, bh, an, bn);
+        const T INa = p[0] * m * m * m * h * (V - T(HH_ENA));
+        const T IK = p[1] * n * n * n * n * (V - T(HH_EK));
+        const T IL = p[2] * (V - T(HH_EL));
+        dy[0] = (d[0] - INa - IK - IL) / T(HH_CM);
+        dy[1] = am * (T(1) - m) - bm * m;
+        dy[2] = ah * (T(1) - h) - bh * h;
+        dy[3] = an * (T(1) - n) - bn * n;
+        break;
+    }
+    case SYS_VAN_DER_POL:
+        dy[0] = y[1];
+        dy[1] = p[0] * (T(1) - y[0] * y[0]) * y[1] - y[0];
+        break;
+    case SYS_LINEAR3:
+        dy[0] = -y[0];
+        dy[1] = -y[1] / T(2);
+        dy[2] = -y[2] / T(3);
+        sys_observables<T>(s, y, p, d, obs, t);
+        break;
+    case SYS_DECAY:
+        for (int i = 0; i < s.n; ++i) dy[i] = -p[i] * y[i];
+        break;
+    case SYS_CALLBACK: {
+        double yd[64], pd[64], dd[16], od[64], dyd[64];
+        for (int i = 0; i < s.n; ++i) yd[i] = double(y[i]);
+        for (int i = 0; i < s.n_params; ++i) pd[i] = double(p[i]);
+        for (int i = 0; i < s.n_drivers; ++i) dd[i] = double(d[i]);
+        s.rhs_cb(yd, pd, dd, od, dyd, double(t));
+        for (int i = 0; i < s.n; ++i) dy[i] = T(dyd[i]);
+        for (int i = 0; i < s.n_obs; ++i) obs[i] = T(od[i]);
+        break;
+    }
+    default:
+        for (int i = 0; i < s.n; ++i) dy[i] = T(0);
+    }
+}
+
+// Analytic Jacobian-vector product jv = J(y) v.
+template <typename T>
+void sys_jvp(const SystemDesc& s, const T* y, const T* p, const T* d,
+             const T* v, T* jv, T t) {
+    switch (s.id) {
+    case SYS_LORENZ:
+        jv[0] = p[0] * (v[1] - v[0]);
+        jv[1] = (p[1] - y[2]) * v[0] - v[1] - y[0] * v[2];
+        jv[2] = y[1] * v[0] + y[0] * v[1] - p[2] * v[2];
+        break;
+    case SYS_LORENZ_JULIA: {
+        const T sigma = T(s.constants[0]), beta = T(s.constants[1]);
+        jv[0] = sigma * (v[1] - v[0]);
+        jv[1] = (p[0] - y[2]) * v[0] - v[1] - y[0] * v[2];
+        jv[2] = y[1] * v[0] + y[0] * v[1] - beta * v[2];
+        break;
+    }
+    case SYS_ROBERTSON: {
+        // f0 = -k1 y0 + k3 y1 y2 ; f1 = k1 y0 - k2 y1^2 - k3 y1 y2 ; f2 = k2 y1^2
+        const T k1 = p[0], k2 = p[1], k3 = p[2];
+        const T d01 = k3 * y[2], d02 = k3 * y[1];
+        jv[0] = -k1 * v[0] + d01 * v[1] + d02 * v[2];
+        jv[1] = k1 * v[0] + (-T(2) * k2 * y[1] - k3 * y[2]) * v[1] - k3 * y[1] * v[2];
+        jv[2] = T(2) * k2 * y[1] * v[1];
+        break;
+    }
+    case SYS_HODGKIN_HUXLEY: {
+        // Central-free analytic derivative of each rate with respect to V.
+        const T V = y[0], m = y[1], h = y[2], n = y[3];
+        T am, bm, ah, bh, an, bn;
+        hh_rates<T>(V, am, bm, ah, bh, an, bn);
+        // d/dV of x/(1-exp(-x/10)) scaled forms
+        auto dratio = [](T a, T x, T scale) {
+            // a = scale * x / (1 - e), e = exp(-x/10)
+            T e = std::exp(-x / T(10));
+            T den = T(1) - e;
+            return scale * (den - x * e / T(10)) / (den * den);
+        };
+        const T dam = dratio(am, V + T(40), T(0.1));
+        const T dan = dratio(an, V + T(55), T(0.01));
+        const T dbm = -bm / T(18);
+        const T dah = -ah / T(20);
+        const T ebh = std::exp(-(V + T(35)) / T(10));
+        const T dbh = (ebh / T(10)) / ((T(1) + ebh) * (T(1) + ebh));
+        const T dbn = -bn / T(80);
+        const T gna = p[0], gk = p[1], gl = p[2];
+        const T m3 = m * m * m, n4 = n * n * n * n;
+        const T dVdV = -(gna * m3 * h + gk * n4 + gl) / T(HH_CM);
+        const T dVdm = -(gna * T(3) * m * m * h * (V - T(HH_ENA))) / T(HH_CM);
+        const T dVdh = -(gna * m3 * (V - T(HH_ENA))) / T(HH_CM);
+        const T dVdn = -(gk * T(4) * n * n * n * (V - T(HH_EK))) / T(HH_CM);
+        jv[0] = dVdV * v[0] + dVdm * v[1] + dVdh * v[2] + dVdn * v[3];
+        jv[1] = (dam * (T(1) - m) - dbm * m) * v[0] - (am + bm) * v[1];
+        jv[2] = (dah * (T(1) - h) - dbh * h) * v[0] - (ah + bh) * v[2];
+        jv[3] = (dan * (T(1) - n) - dbn * n) * v[0] - (an + bn) * v[3];
+        break;
+    }
+    case SYS_VAN_DER_POL:
+        jv[0] = v[1];
+        jv[1] = (-T(2) * p[0] * y[0] * y[1] - T(1)) * v[0] + p[0] * (T(1) - y[0] * y[0]) * v[1];
+        break;
+    case SYS_LINEAR3:
+        jv[0] = -v[0];
+        jv[1] = -v[1] / T(2);
+        jv[2] = -v[2] / T(3);
+        break;
+    case SYS_DECAY:
+        for (int i = 0; i < s.n; ++i) jv[i] = -p[i] * v[i];
+        break;
+    case SYS_CALLBACK: {
+        double yd[64], pd[64], dd[16], vd[64], jd[64];
+        for (int i = 0; i < s.n; ++i) { yd[i] = double(y[i]); vd[i] = double(v[i]); }
+        for (int i = 0; i < s.n_params; ++i) pd[i] = double(p[i]);
+        for (int i = 0; i < s.n_drivers; ++i) dd[i] = double(d[i]);
+        s.jvp_cb(yd, pd, dd, vd, jd, double(t));
+        for (int i = 0; i < s.n; ++i) jv[i] = T(jd[i]);
+        break;
+    }
+    default:
+        for (int i = 0; i < s.n; ++i) jv[i] = T(0);
+    }
+}
+
+}  // namespace oracle
