@@ -18,6 +18,7 @@ module: ``cubie_integrate`` (one lane per system) and
 from __future__ import annotations
 
 import math
+import os
 import warnings
 from dataclasses import dataclass
 from typing import Any, Dict, Optional
@@ -33,6 +34,14 @@ from ..runtime import kcache, native
 
 KERNEL_MODES = ("plain", "persistent")
 PRECISION_MODES = ("plain", "compensated")
+
+
+class CompileOnly(Exception):
+    """Raised by ``run`` when CUBIE_COMPILE_ONLY=1: the code object is now cached."""
+
+
+def _compile_only() -> bool:
+    return os.environ.get("CUBIE_COMPILE_ONLY", "") == "1"
 
 
 def _torch():
@@ -57,7 +66,7 @@ class BatchSolverKernel:
                  algorithm_settings=None, output_settings=None, loop_settings=None,
                  kernel_mode: str = "plain", precision_mode: str = "plain",
                  fp_contract: Optional[bool] = None, block_size: int = 256,
-                 blocks_per_cu: int = 4):
+                 blocks_per_cu: int = 4, deterministic_pow: bool = False):
         self.system = system
         self.precision = system.precision
         if kernel_mode not in KERNEL_MODES:
@@ -71,6 +80,7 @@ class BatchSolverKernel:
         self.block_size = int(block_size)
         self.blocks_per_cu = int(blocks_per_cu)
         self.fp_contract = bool(fp_contract) if fp_contract is not None else False
+        self.deterministic_pow = bool(deterministic_pow)
         self.driver_interpolator = None
         self._algorithm_arg = algorithm
         self._step_settings = dict(step_control_settings or {})
@@ -170,7 +180,8 @@ class BatchSolverKernel:
             elif k in ALL_OUTPUT_FUNCTION_PARAMETERS:
                 self._output_settings[k] = v
                 rec.add(k)
-            elif k in ("kernel_mode", "precision_mode", "block_size", "blocks_per_cu", "fp_contract"):
+            elif k in ("kernel_mode", "precision_mode", "block_size", "blocks_per_cu", "fp_contract",
+                       "deterministic_pow"):
                 setattr(self, k, v)
                 rec.add(k)
         if rec:
@@ -202,14 +213,15 @@ class BatchSolverKernel:
     def spec_for(self, duration: float, per_run_rtol: bool = False) -> KernelSpec:
         timing = self.timing.with_duration(duration)
         key = (timing.summarise_every, timing.sample_summaries_every, per_run_rtol,
-               self.precision_mode, self.fp_contract, self.block_size)
+               self.precision_mode, self.fp_contract, self.block_size, self.deterministic_pow)
         if key not in self._spec_cache:
             self._spec_cache[key] = KernelSpec(
                 system=self.system, precision=self.precision, algorithm=self.algorithm,
                 controller=self.controller, timing=timing, outputs=self.output_config,
                 drivers=self.driver_interpolator, implicit=self.implicit,
                 compensated=self.precision_mode == "compensated", per_run_rtol=per_run_rtol,
-                fp_contract=self.fp_contract, block_size=self.block_size)
+                fp_contract=self.fp_contract, block_size=self.block_size,
+                deterministic_pow=self.deterministic_pow)
         return self._spec_cache[key]
 
     def module(self, spec: KernelSpec, kernel_mode: Optional[str] = None):
@@ -244,6 +256,11 @@ class BatchSolverKernel:
 
         Returns (DeviceOutputs, kernel_ms or None, launch_info).
         """
+        if _compile_only():
+            dur = float(np.float64(duration))
+            spec = self.spec_for(dur, rtol_per_run is not None)
+            kcache.get_code(spec.source(), spec.compile_options())
+            raise CompileOnly(spec)
         torch = _torch()
         prec = self.precision
         tdt = torch.float32 if prec is np.float32 else torch.float64

@@ -18,6 +18,14 @@ namespace cubie {
 
 enum { CTRL_FIXED = 0, CTRL_I = 1, CTRL_PI = 2, CTRL_PID = 3, CTRL_GUSTAFSSON = 4 };
 
+// Controller powers: the device library's pow, or (parity builds) the
+// deterministic pow shared bit for bit with the host oracle.
+template <typename Cfg, typename T>
+__device__ __forceinline__ T cpow(T x, T e) {
+    if constexpr (Cfg::deterministic_pow) return cubie_det::pow_pos<T>(x, e);
+    else return dpow(x, e);
+}
+
 template <typename Cfg>
 struct Controller {
     using real = typename Cfg::real;
@@ -69,13 +77,13 @@ struct Controller {
         const nreal cur = nreal(dt);
         nreal gain;
         if constexpr (Cfg::controller == CTRL_I) {
-            nreal gaintmp = safety * dpow(nrm2, -nreal(Cfg::order_exponent));
+            nreal gaintmp = safety * cpow<Cfg>(nrm2, -nreal(Cfg::order_exponent));
             gain = clampv(gaintmp, nreal(Cfg::min_gain), nreal(Cfg::max_gain));
         } else if constexpr (Cfg::controller == CTRL_PI) {
             nreal err_prev = history[0];
-            nreal pgain = dpow(nrm2, -nreal(Cfg::kp_exp));
+            nreal pgain = cpow<Cfg>(nrm2, -nreal(Cfg::kp_exp));
             nreal err_source = err_prev > nreal(0) ? err_prev : nrm2;
-            nreal igain = dpow(err_source, -nreal(Cfg::ki_exp));
+            nreal igain = cpow<Cfg>(err_source, -nreal(Cfg::ki_exp));
             nreal gain_new = safety * pgain * igain;
             gain = clampv(gain_new, nreal(Cfg::min_gain), nreal(Cfg::max_gain));
             history[0] = commit ? nrm2 : err_prev;
@@ -83,8 +91,8 @@ struct Controller {
             nreal err_prev = history[0], err_prev_prev = history[1];
             nreal eps = err_prev > nreal(0) ? err_prev : nrm2;
             nreal epps = err_prev_prev > nreal(0) ? err_prev_prev : eps;
-            nreal gain_new = safety * dpow(nrm2, -nreal(Cfg::kp_exp)) *
-                             dpow(eps, -nreal(Cfg::ki_exp)) * dpow(epps, -nreal(Cfg::kd_exp));
+            nreal gain_new = safety * cpow<Cfg>(nrm2, -nreal(Cfg::kp_exp)) *
+                             cpow<Cfg>(eps, -nreal(Cfg::ki_exp)) * cpow<Cfg>(epps, -nreal(Cfg::kd_exp));
             gain = clampv(gain_new, nreal(Cfg::min_gain), nreal(Cfg::max_gain));
             history[1] = commit ? err_prev : err_prev_prev;
             history[0] = commit ? nrm2 : err_prev;
@@ -95,9 +103,9 @@ struct Controller {
             nreal tmp = nreal(Cfg::gain_numerator) / denom;
             nreal fac = safety < tmp ? safety : tmp;
             nreal expo = nreal(Cfg::order_exponent);
-            nreal gain_basic = fac * dpow(nrm2, -expo);
+            nreal gain_basic = fac * cpow<Cfg>(nrm2, -expo);
             nreal ratio = nrm2 * nrm2 / err_prev;
-            nreal gain_gus = safety * (cur / dt_prev) * dpow(ratio, -expo);
+            nreal gain_gus = safety * (cur / dt_prev) * cpow<Cfg>(ratio, -expo);
             gain = gain_gus < gain_basic ? gain_gus : gain_basic;
             gain = (accept && dt_prev > nreal(1e-16)) ? gain : gain_basic;
             gain = clampv(gain, nreal(Cfg::min_gain), nreal(Cfg::max_gain));

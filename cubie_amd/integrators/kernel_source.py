@@ -32,7 +32,7 @@ CSRC = os.path.join(os.path.dirname(_HERE), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include")
 
 DEVICE_HEADERS = [
-    "common.hpp", "drivers.hpp", "controllers.hpp", "outputs.hpp", "linear.hpp",
+    "detmath.hpp", "common.hpp", "drivers.hpp", "controllers.hpp", "outputs.hpp", "linear.hpp",
     "erk.hpp", "dirk.hpp", "firk.hpp", "rosenbrock.hpp", "loop.hpp",
 ]
 
@@ -70,6 +70,7 @@ class KernelSpec:
     compensated: bool = False
     per_run_rtol: bool = False
     fp_contract: bool = False
+    deterministic_pow: bool = False
     block_size: int = 256
 
     # -- helpers ---------------------------------------------------------------
@@ -106,6 +107,7 @@ class KernelSpec:
         a(f"  static constexpr bool adaptive = {str(adaptive).lower()};")
         a(f"  static constexpr bool compensated = {str(self.compensated).lower()};")
         a(f"  static constexpr bool per_run_rtol = {str(self.per_run_rtol).lower()};")
+        a(f"  static constexpr bool deterministic_pow = {str(self.deterministic_pow).lower()};")
         # controller
         a(f"  static constexpr int controller = {ctrl.id};")
         a(f"  static constexpr real atol[{max(n, 1)}] = {c_array(ctrl.atol, p)};")

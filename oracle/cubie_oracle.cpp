@@ -33,6 +33,7 @@
 #include <limits>
 
 #include "systems.hpp"
+#include "../cubie_amd/csrc/device/detmath.hpp"
 
 namespace oracle {
 
@@ -130,6 +131,8 @@ struct OracleConfig {
     rhs_cb_t rhs_cb;
     obs_cb_t obs_cb;
     jvp_cb_t jvp_cb;
+    // controller powers through the deterministic pow (detmath.hpp)
+    int32_t deterministic_pow;
 };
 
 }  // extern "C"
@@ -330,8 +333,13 @@ struct ERKStepper {
 template <typename T>
 static inline T clampv(T v, T lo, T hi) { return std::max(lo, std::min(v, hi)); }
 
+// Library pow (the reference's CPU path) or the deterministic pow shared bit
+// for bit with the HIP kernels' parity builds.
+static bool g_det_pow = false;
 template <typename T>
-static inline T tpow(T x, T e) { return std::pow(x, e); }
+static inline T tpow(T x, T e) {
+    return g_det_pow ? cubie_det::pow_pos<T>(x, e) : std::pow(x, e);
+}
 
 template <typename T>
 struct ControllerState {
@@ -960,6 +968,7 @@ int oracle_solve(const oracle::OracleConfig* cfg, int64_t n_runs,
     oracle::Outputs out{state_out, obs_out, state_summ_out, obs_summ_out,
                         counters_out, status_out, n_runs};
     if (run_end > n_runs) run_end = n_runs;
+    oracle::g_det_pow = cfg->deterministic_pow != 0;
     if (cfg->precision_bits == 32)
         return oracle::dispatch_family<float>(cfg, n_runs, inits, params, out, run_begin, run_end);
     return oracle::dispatch_family<double>(cfg, n_runs, inits, params, out, run_begin, run_end);

@@ -82,12 +82,14 @@ class OracleConfig(ctypes.Structure):
         ("krylov_atol_s", ctypes.c_double), ("krylov_rtol_s", ctypes.c_double),
         ("krylov_residual_reduction", ctypes.c_double), ("krylov_floor", ctypes.c_double),
         ("rhs_cb", RHS_CB), ("obs_cb", OBS_CB), ("jvp_cb", JVP_CB),
+        ("deterministic_pow", ctypes.c_int32),
     ]
 
 
 def build(force: bool = False) -> str:
     src = os.path.join(HERE, "cubie_oracle.cpp")
-    deps = [src, os.path.join(HERE, "systems.hpp"), os.path.join(HERE, "oracle_implicit.hpp")]
+    deps = [src, os.path.join(HERE, "systems.hpp"), os.path.join(HERE, "oracle_implicit.hpp"),
+            os.path.join(os.path.dirname(HERE), "cubie_amd", "csrc", "device", "detmath.hpp")]
     if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(d) for d in deps):
         subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
@@ -245,6 +247,7 @@ class OracleRun:
             cfg.rhs_cb = RHS_CB()
             cfg.obs_cb = OBS_CB()
             cfg.jvp_cb = JVP_CB()
+        cfg.deterministic_pow = int(getattr(k, "deterministic_pow", False))
         self.cfg = cfg
         self.keep = keep
         self.kernel = k
