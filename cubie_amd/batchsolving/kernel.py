@@ -131,10 +131,10 @@ class BatchSolverKernel:
         ctrl = self.controller
         if self.algorithm.is_implicit:
             # SingleIntegratorRunCore._apply_inner_tolerance_defaults
-            im.krylov_atol = float(ctrl.atol.min())
-            im.krylov_rtol = float(ctrl.rtol.min())
-            im.newton_atol = float(ctrl.atol.min()) / 10.0
-            im.newton_rtol = float(ctrl.rtol.min()) / 10.0
+            im.krylov_atol = ctrl.atol.copy()
+            im.krylov_rtol = ctrl.rtol.copy()
+            im.newton_atol = ctrl.atol / 10.0
+            im.newton_rtol = ctrl.rtol / 10.0
             floor = float(ctrl.rtol.min())
             if ctrl.is_adaptive and floor > 0.0:
                 if self.algorithm.is_linear:
@@ -142,10 +142,13 @@ class BatchSolverKernel:
                 im.krylov_residual_reduction = floor
             else:
                 im.krylov_residual_reduction = float(np.finfo(self.precision).eps)
-        for key in ("newton_atol", "newton_rtol", "krylov_atol", "krylov_rtol",
-                    "krylov_residual_reduction"):
+        for key in ("newton_atol", "newton_rtol", "krylov_atol", "krylov_rtol"):
             if a.get(key) is not None:
-                setattr(im, key, float(np.min(a[key])))
+                setattr(im, key, np.asarray(a[key], dtype=np.float64))
+        if a.get("krylov_residual_reduction") is not None:
+            im.krylov_residual_reduction = float(a["krylov_residual_reduction"])
+        if a.get("attempt_dense_prediction") is not None:
+            im.attempt_dense_prediction = bool(a["attempt_dense_prediction"])
         if a.get("krylov_residual_floor") is not None:
             im.krylov_residual_floor = float(a["krylov_residual_floor"])
         for key in ("newton_max_iters", "krylov_max_iters", "preconditioner_order"):

@@ -1,2 +1,245 @@
-// Placeholder: the dirk component is added with the implicit families.
+// Diagonally implicit Runge-Kutta step (SDIRK / ESDIRK tableaus).
+//
+// Follows the reference's DIRKStep.step (integrators/algorithms/
+// generic_dirk.py:515-857): explicit couplings are streamed into per-stage
+// accumulators exactly as in the ERK step, each implicit stage solves
+// u = h f(t_i, base_i + a_ii u) by Newton-Krylov (linear.hpp) starting from
+// the carried or predicted increment, the stage state is base_i + a_ii u,
+// and the solution / error are accumulated from the stage derivatives (or
+// copied when b / b_hat equals a row of A). First-same-as-last reuse follows
+// the wave vote. With dense prediction the previous step's converged
+// increments are read ahead to the new stage times (stage_predictors.py
+// DenseStagePredictor: ratio polynomials evaluated in Horner form, applied
+// after an accepted step whose size ratio is within the tableau's ceiling).
 #pragma once
+
+namespace cubie {
+
+template <typename Cfg, typename Sys>
+struct DIRKStep {
+    static constexpr int n = Cfg::n;
+    static constexpr int s = Cfg::stages;
+    // persistent state
+    Vec<real, n> stage_rhs;
+    Vec<real, n> stage_increment;
+    Vec<real, (Cfg::dense_prediction ? s * n : 1)> history;
+    real previous_step_size;
+    real theta;                       // Newton contraction estimate
+
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int i = 0; i < n; ++i) { stage_rhs[i] = real(0); stage_increment[i] = real(0); }
+#pragma unroll
+        for (int i = 0; i < (Cfg::dense_prediction ? s * n : 1); ++i) history[i] = real(0);
+        previous_step_size = real(0);
+        theta = real(0);
+    }
+
+    __device__ __forceinline__ void predict(real step_ratio, bool apply) {
+        if constexpr (Cfg::dense_prediction) {
+            constexpr int P = Cfg::pred_powers;
+            constexpr int R = Cfg::pred_rows;
+            constexpr int F = Cfg::pred_first;
+            real transform[R * s];
+#pragma unroll
+            for (int e = 0; e < R * s; ++e) {
+                real acc = real(0);
+#pragma unroll
+                for (int p = P - 1; p >= 0; --p) acc = acc * step_ratio + Cfg::pred_coeffs[p][e];
+                transform[e] = acc * step_ratio;
+            }
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                real prev[s];
+#pragma unroll
+                for (int k = 0; k < s; ++k) prev[k] = history[k * n + i];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    real acc = real(0);
+#pragma unroll
+                    for (int k = 0; k < s; ++k) acc += transform[r * s + k] * prev[k];
+                    const int target = (F + r) * n + i;
+                    history[target] = apply ? acc : history[target];
+                }
+            }
+        }
+    }
+
+    template <typename Ctx>
+    __device__ __forceinline__ int step(Ctx& ctx, const Vec<real, n>& state, Vec<real, n>& proposed,
+                                        Vec<real, Cfg::n_error>& error, real dt, real time,
+                                        double /*t64*/, bool first_step, bool prev_accepted,
+                                        int* counters) {
+        const real current_time = time;
+        const real end_time = current_time + dt;
+        Vec<real, (s > 1 ? (s - 1) * n : 1)> acc;
+        Vec<real, n> stage_base;
+        int status = ST_SUCCESS;
+        // Like the reference, each stage solve overwrites the step counters, so
+        // they report the last implicit stage (Newton iterations, Krylov iterations).
+#pragma unroll
+        for (int i = 0; i < (s > 1 ? (s - 1) * n : 1); ++i) acc[i] = real(0);
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            if constexpr (Cfg::has_error && Cfg::accumulates_error) error[i] = real(0);
+        }
+
+        if constexpr (Cfg::dense_prediction) {
+            const real previous_dt = previous_step_size;
+            previous_step_size = dt;
+            const real safe_prev = previous_dt > real(0) ? previous_dt : dt;
+            const real step_ratio = dt / safe_prev;
+            const bool apply = (!first_step) && prev_accepted && (step_ratio <= Cfg::pred_max_ratio);
+            predict(step_ratio, apply);
+        }
+
+        bool use_cached = false;
+        if constexpr (Cfg::fsal && s > 1) {
+            if (!first_step) use_cached = wave_all(prev_accepted);
+        }
+
+        real stage_time = current_time + dt * Cfg::c[0];
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            stage_base[i] = state[i];
+            if constexpr (Cfg::accumulates_output) proposed[i] = real(0);
+        }
+        if (!use_cached) {
+            if constexpr (Cfg::c[0] == 0.0) {
+#pragma unroll
+                for (int i = 0; i < Cfg::n_drivers; ++i) ctx.proposed_drivers[i] = ctx.drivers[i];
+            } else {
+                ctx.drv.eval(stage_time, ctx.proposed_drivers.data());
+            }
+            if constexpr (Cfg::A[0][0] != 0.0) {
+                if constexpr (Cfg::dense_prediction) {
+#pragma unroll
+                    for (int i = 0; i < n; ++i) stage_increment[i] = history[i];
+                }
+                int cnt[2] = {0, 0};
+                status |= solve_stage(ctx, stage_base, state, stage_time, dt, Cfg::A[0][0], cnt);
+                counters[0] = cnt[0];
+                counters[1] = cnt[1];
+                if constexpr (Cfg::dense_prediction) {
+#pragma unroll
+                    for (int i = 0; i < n; ++i) history[i] = stage_increment[i];
+                }
+#pragma unroll
+                for (int i = 0; i < n; ++i) stage_base[i] += Cfg::A[0][0] * stage_increment[i];
+            }
+            if constexpr (Cfg::n_obs > 0)
+                Sys::observables(stage_base.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                                 ctx.proposed_observables.data(), stage_time);
+            Sys::dxdt(stage_base.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                      ctx.proposed_observables.data(), stage_rhs.data(), stage_time);
+        }
+        if constexpr (Cfg::dense_prediction && Cfg::A[0][0] == 0.0) {
+#pragma unroll
+            for (int i = 0; i < n; ++i) history[i] = dt * stage_rhs[i];
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            const real rv = stage_rhs[i];
+            if constexpr (Cfg::accumulates_output) proposed[i] += Cfg::b[0] * rv;
+            else if constexpr (Cfg::b_row == 0) proposed[i] = stage_base[i];
+            if constexpr (Cfg::has_error) {
+                if constexpr (Cfg::accumulates_error) error[i] += Cfg::d[0] * rv;
+                else if constexpr (Cfg::b_hat_row == 0) error[i] = stage_base[i];
+            }
+        }
+
+#pragma unroll
+        for (int prev = 0; prev < s - 1; ++prev) {
+            const int stage_idx = prev + 1;
+#pragma unroll
+            for (int succ = prev; succ < s - 1; ++succ) {
+                if (Cfg::A[succ + 1][prev] != 0.0) {
+#pragma unroll
+                    for (int i = 0; i < n; ++i) acc[succ * n + i] += Cfg::A[succ + 1][prev] * stage_rhs[i];
+                }
+            }
+            stage_time = current_time + dt * Cfg::c[stage_idx];
+            ctx.drv.eval(stage_time, ctx.proposed_drivers.data());
+#pragma unroll
+            for (int i = 0; i < n; ++i) stage_base[i] = acc[prev * n + i] * dt + state[i];
+            const real diag = Cfg::A[stage_idx][stage_idx];
+            if (diag != real(0)) {
+                if constexpr (Cfg::dense_prediction) {
+                    const int src = Cfg::pred_source[stage_idx];
+#pragma unroll
+                    for (int i = 0; i < n; ++i) stage_increment[i] = history[src * n + i];
+                }
+                int cnt[2] = {0, 0};
+                status |= solve_stage(ctx, stage_base, state, stage_time, dt, diag, cnt);
+                counters[0] = cnt[0];
+                counters[1] = cnt[1];
+                if constexpr (Cfg::dense_prediction) {
+#pragma unroll
+                    for (int i = 0; i < n; ++i) history[stage_idx * n + i] = stage_increment[i];
+                }
+#pragma unroll
+                for (int i = 0; i < n; ++i) stage_base[i] += diag * stage_increment[i];
+            }
+            if constexpr (Cfg::n_obs > 0)
+                Sys::observables(stage_base.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                                 ctx.proposed_observables.data(), stage_time);
+            Sys::dxdt(stage_base.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                      ctx.proposed_observables.data(), stage_rhs.data(), stage_time);
+            if constexpr (Cfg::dense_prediction && Cfg::dirk_has_later_explicit) {
+                if (diag == real(0)) {
+#pragma unroll
+                    for (int i = 0; i < n; ++i) history[stage_idx * n + i] = dt * stage_rhs[i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const real inc = stage_rhs[i];
+                if constexpr (Cfg::accumulates_output) {
+                    proposed[i] += Cfg::b[stage_idx] * inc;
+                } else {
+                    if (Cfg::b_row == stage_idx) proposed[i] = stage_base[i];
+                }
+                if constexpr (Cfg::has_error) {
+                    if constexpr (Cfg::accumulates_error) {
+                        error[i] += Cfg::d[stage_idx] * inc;
+                    } else {
+                        if (Cfg::b_hat_row == stage_idx) error[i] = stage_base[i];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            if constexpr (Cfg::accumulates_output) {
+                proposed[i] *= dt;
+                proposed[i] += state[i];
+            }
+            if constexpr (Cfg::has_error) {
+                if constexpr (Cfg::accumulates_error) error[i] *= dt;
+                else error[i] = proposed[i] - error[i];
+            }
+        }
+        ctx.drv.eval(end_time, ctx.proposed_drivers.data());
+        if constexpr (Cfg::n_obs > 0)
+            Sys::observables(proposed.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                             ctx.proposed_observables.data(), end_time);
+        return status;
+    }
+
+    template <typename Ctx>
+    __device__ __forceinline__ int solve_stage(Ctx& ctx, const Vec<real, n>& base, const Vec<real, n>& state,
+                                               real t, real h, real a, int* cnt) {
+        DirkStageProblem<Cfg, Sys> P;
+        P.u = stage_increment.data();
+        P.base = base.data();
+        P.step_start = state.data();
+        P.params = ctx.params.data();
+        P.drivers = ctx.proposed_drivers.data();
+        P.t = t;
+        P.h = h;
+        P.a = a;
+        return newton_solve<Cfg>(P, theta, cnt);
+    }
+};
+
+}  // namespace cubie

@@ -1,2 +1,351 @@
-// Placeholder: the linear component is added with the implicit families.
+// Matrix-free implicit-stage machinery: weighted norms, the linearised stage
+// operator, the truncated Neumann preconditioner, minimal-residual /
+// steepest-descent and BiCGSTAB linear solvers, and the Newton iteration.
+//
+// Behaviour follows the reference:
+//   operator   beta*v - gamma*a*h*J(base + a*u) v
+//              (odesystems/symbolic/codegen/linear_operators.py:118-149, 241-298)
+//   residual   beta*u - gamma*h*f(t, base + a*u)
+//              (codegen/nonlinear_residuals.py:59-87)
+//   Neumann    out = v; repeat order: out = v + (gamma/beta)*h*a*J(out);
+//              out /= beta (codegen/preconditioners.py:85-142, 278-307)
+//   MR / SD    integrators/matrix_free_solvers/linear_solver.py:157-563
+//   BiCGSTAB   matrix_free_solvers/bicgstab_solver.py:226-672, with the same
+//              value clamps (1e16 / 1e150) and breakdown guards (1e-30 / 1e-200)
+//   stopping   ||r||_w <= floor + reduction * ||b||_w
+//              (linear_solver_base.py:59-140)
+//   Newton     NLNewton contraction tests, matrix_free_solvers/newton_krylov.py:
+//              276-518 (kappa 0.01, first-iterate bound 1e-5, theta decay 0.3,
+//              divergence 2, stagnation band 100*sqrt(eps), warm-started theta)
+//   norms      integrators/norms.py:125-432 (tolerance floor 1e-16)
+// beta = gamma = 1 (the reference's ImplicitStepConfig defaults) and the mass
+// matrix is the identity.
+//
+// A stage "problem" type supplies the width W and the device callbacks
+// (operator, preconditioner, residual, norms) over register-resident
+// vectors; the solvers below are written once against that interface and
+// serve DIRK stages (W = n), coupled FIRK stages (W = s*n) and Rosenbrock
+// stages (linear solves only).
 #pragma once
+
+namespace cubie {
+
+enum { LIN_MR = 0, LIN_SD = 1, LIN_BICGSTAB = 2 };
+
+template <typename real>
+struct SolverConsts;
+template <>
+struct SolverConsts<double> {
+    static constexpr double clamp = 1e150;
+    static constexpr double breakdown = 1e-200;
+    static constexpr double huge = 1.7976931348623157e308;
+    static constexpr double stagnation_eps = 1.4901161193847656e-06;  // 100*sqrt(eps)
+};
+template <>
+struct SolverConsts<float> {
+    static constexpr float clamp = 1e16f;
+    static constexpr float breakdown = 1e-30f;
+    static constexpr float huge = 3.4028234663852886e38f;
+    static constexpr float stagnation_eps = 0.034526698300124390f;      // 100*sqrt(eps)
+};
+
+template <typename real>
+__device__ __forceinline__ real clampc(real x) {
+    constexpr real c = SolverConsts<real>::clamp;
+    x = x > c ? c : x;
+    x = x < -c ? -c : x;
+    return x;
+}
+template <typename real>
+__device__ __forceinline__ real clamp_pos(real x) {
+    constexpr real c = SolverConsts<real>::clamp;
+    return x > c ? c : x;
+}
+
+// Preconditioned minimal-residual / steepest-descent solve of A x = rhs.
+// rhs is overwritten by the running residual; x holds the initial guess.
+template <typename Cfg, typename Prob>
+__device__ __forceinline__ int mr_solve(Prob& P, real* rhs, real* x, bool x_is_zero, int& iters) {
+    constexpr int W = Prob::W;
+    const real rhs_norm2 = P.krylov_norm2(rhs);
+    const real tol = Cfg::krylov_floor + Cfg::krylov_reduction * real(dsqrt(rhs_norm2));
+    const real tol2 = tol * tol;
+    real temp[W], z[W];
+    if (!x_is_zero) {
+        P.op(x, temp);
+#pragma unroll
+        for (int i = 0; i < W; ++i) rhs[i] = rhs[i] - temp[i];
+    }
+    real acc = P.krylov_norm2(rhs);
+    bool converged = acc <= tol2;
+    int it = 0;
+    for (int k = 0; k < Cfg::krylov_max_iters; ++k) {
+        if (converged) break;
+        ++it;
+        P.precond(rhs, z);
+        P.op(z, temp);
+        real num = real(0), den = real(0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            if constexpr (Cfg::linear_solver == LIN_SD) {
+                num += rhs[i] * z[i];
+                den += temp[i] * z[i];
+            } else {
+                num += temp[i] * rhs[i];
+                den += temp[i] * temp[i];
+            }
+        }
+        const real alpha = den != real(0) ? num / den : real(0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            x[i] += alpha * z[i];
+            rhs[i] -= alpha * temp[i];
+        }
+        acc = P.krylov_norm2(rhs);
+        converged = converged || (acc <= tol2);
+    }
+    iters = it;
+    return converged ? ST_SUCCESS : ST_LINEAR_MAX_ITERS;
+}
+
+// Preconditioned BiCGSTAB with the reference's clamps and breakdown guards.
+template <typename Cfg, typename Prob>
+__device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool x_is_zero, int& iters) {
+    constexpr int W = Prob::W;
+    constexpr real brk = SolverConsts<real>::breakdown;
+    constexpr real dclamp = SolverConsts<real>::clamp;
+    const real rhs_norm2 = P.krylov_norm2(rhs);
+    const real tol = Cfg::krylov_floor + Cfg::krylov_reduction * real(dsqrt(rhs_norm2));
+    const real tol2 = tol * tol;
+    real r0_hat[W], p[W], v[W], tmp[W], s_hat[W];
+    if (!x_is_zero) {
+        P.op(x, tmp);
+    } else {
+#pragma unroll
+        for (int i = 0; i < W; ++i) tmp[i] = real(0);
+    }
+    real rho_prev = real(0);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const real ax = clampc(tmp[i]);
+        const real ri = rhs[i] - ax;
+        rhs[i] = ri;
+        r0_hat[i] = ri;
+        p[i] = clampc(ri);
+        rho_prev += clamp_pos(ri * ri);
+    }
+    real acc = P.krylov_norm2(rhs);
+    bool converged = acc <= tol2;
+    bool broken = false;
+    bool finished = converged;
+    int it = 0;
+    for (int k = 0; k < Cfg::krylov_max_iters; ++k) {
+        if (finished) break;
+        ++it;
+        P.precond(p, tmp);
+#pragma unroll
+        for (int i = 0; i < W; ++i) tmp[i] = clampc(tmp[i]);
+        P.op(tmp, v);
+        real dot_r0v = real(0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            v[i] = clampc(v[i]);
+            dot_r0v += clampc(r0_hat[i] * v[i]);
+        }
+        const bool alpha_overflow = dabs(rho_prev) > dabs(dot_r0v) * dclamp;
+        broken = broken || (!finished && alpha_overflow);
+        finished = converged || broken;
+        const real alpha = (dot_r0v != real(0) && !alpha_overflow) ? rho_prev / dot_r0v : real(0);
+        const real alpha_eff = finished ? real(0) : alpha;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            x[i] = x[i] + alpha_eff * tmp[i];
+            rhs[i] = rhs[i] - alpha_eff * v[i];
+        }
+        acc = P.krylov_norm2(rhs);
+        converged = converged || (acc <= tol2);
+        finished = converged || broken;
+        P.precond(rhs, s_hat);
+#pragma unroll
+        for (int i = 0; i < W; ++i) s_hat[i] = clampc(s_hat[i]);
+        P.op(s_hat, tmp);
+        real dot_ts = real(0), dot_tt = real(0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const real ti = clampc(tmp[i]);
+            tmp[i] = ti;
+            dot_ts += clampc(ti * rhs[i]);
+            dot_tt += clamp_pos(ti * ti);
+        }
+        const bool omega_overflow = dabs(dot_ts) > dot_tt * dclamp;
+        const real omega = (dot_tt != real(0) && !omega_overflow) ? dot_ts / dot_tt : real(0);
+        const real omega_eff = finished ? real(0) : omega;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            x[i] = x[i] + omega_eff * s_hat[i];
+            rhs[i] = rhs[i] - omega_eff * tmp[i];
+        }
+        acc = P.krylov_norm2(rhs);
+        converged = converged || (acc <= tol2);
+        real rho_new = real(0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) rho_new += clampc(r0_hat[i] * rhs[i]);
+        const bool rho_bad = dabs(rho_new) < brk;
+        const bool omega_bad = dabs(omega) < brk;
+        broken = broken || (!converged && (rho_bad || omega_bad));
+        finished = converged || broken;
+        const bool beta_overflow = (dabs(rho_new) > dabs(rho_prev) * dclamp) ||
+                                   (dabs(alpha) > dabs(omega) * dclamp);
+        broken = broken || (!finished && beta_overflow);
+        finished = converged || broken;
+        const real beta = !finished ? (rho_new / rho_prev) * (alpha / omega) : real(0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            real pi = !finished ? rhs[i] + beta * (p[i] - omega * v[i]) : p[i];
+            p[i] = clampc(pi);
+        }
+        rho_prev = !finished ? rho_new : rho_prev;
+    }
+    iters = it;
+    return converged ? ST_SUCCESS : (broken ? ST_BICGSTAB_BREAKDOWN : ST_LINEAR_MAX_ITERS);
+}
+
+template <typename Cfg, typename Prob>
+__device__ __forceinline__ int linear_solve(Prob& P, real* rhs, real* x, bool x_is_zero, int& iters) {
+    if constexpr (Cfg::linear_solver == LIN_BICGSTAB) return bicgstab_solve<Cfg>(P, rhs, x, x_is_zero, iters);
+    else return mr_solve<Cfg>(P, rhs, x, x_is_zero, iters);
+}
+
+// Newton iteration on the stage increment P.u (NLNewton acceptance rules).
+// theta_store carries the contraction estimate between solves.
+template <typename Cfg, typename Prob>
+__device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* counters) {
+    constexpr int W = Prob::W;
+    const real kappa = real(0.01);
+    const real first_bound = real(1.0e-5);
+    const real theta_decay = real(0.3);
+    const real theta_div = real(2.0);
+    constexpr real huge = SolverConsts<real>::huge;
+    constexpr real stag_eps = SolverConsts<real>::stagnation_eps;
+    real prev_theta = theta_store > real(0) ? theta_store : real(1);
+    real ndz_prev = real(0);
+    bool converged = false, failed = false;
+    int last_lin = ST_SUCCESS;
+    int iters = 0, krylov_total = 0, iteration = 0;
+    real residual[W], delta[W];
+    for (int k = 0; k < Cfg::newton_max_iters; ++k) {
+        if (converged || failed) break;
+        ++iteration;
+        P.residual(residual);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            residual[i] = -residual[i];
+            delta[i] = real(0);
+        }
+        int kit = 0;
+        const int lin = linear_solve<Cfg>(P, residual, delta, true, kit);
+        krylov_total += kit;
+        last_lin = lin;
+        ++iters;
+        const real norm2_dz = P.correction_norm2(delta);
+        const real ndz = real(dsqrt(norm2_dz));
+        const bool judged = lin == ST_SUCCESS;
+        const bool history = ndz_prev > real(0);
+        const real ndz_prev_safe = history ? ndz_prev : real(1);
+        const real theta = history ? pmax(theta_decay * prev_theta, ndz / ndz_prev_safe) : prev_theta;
+        const bool small_first = (iteration == 1) && (ndz < first_bound);
+        const bool eta_accept = (theta < real(1)) && (theta * ndz < kappa * (real(1) - theta));
+        const bool nonfinite = !(norm2_dz <= huge);
+        const bool stagnant = judged && history && (dabs(theta - real(1)) <= stag_eps);
+        const bool diverging = judged && ((history && (theta > theta_div)) || nonfinite);
+        const bool converged_stagnant = stagnant && (ndz <= real(1)) && !diverging;
+        const bool failed_now = diverging || (stagnant && (ndz > real(1)));
+        failed = failed || failed_now;
+        const bool commit = judged && !failed_now && !converged_stagnant;
+#pragma unroll
+        for (int i = 0; i < W; ++i) P.u[i] = commit ? P.u[i] + delta[i] : P.u[i];
+        converged = converged || converged_stagnant || (commit && (eta_accept || small_first));
+        ndz_prev = commit ? ndz : real(0);
+        prev_theta = (judged && history) ? theta : prev_theta;
+    }
+    theta_store = converged ? prev_theta : real(1);
+    int fail_bits = failed ? ST_NEWTON_DIVERGENCE : ST_NEWTON_MAX_ITERS;
+    fail_bits = (last_lin != ST_SUCCESS) ? (fail_bits | last_lin) : fail_bits;
+    counters[0] = iters;
+    counters[1] = krylov_total;
+    return converged ? ST_SUCCESS : fail_bits;
+}
+
+// Single-stage (DIRK) implicit problem: unknown u, stage state base + a*u.
+template <typename Cfg, typename Sys>
+struct DirkStageProblem {
+    static constexpr int n = Cfg::n;
+    static constexpr int W = n;
+    real* u;               // stage increment (Newton iterate)
+    const real* base;      // stage base
+    const real* step_start;
+    const real* params;
+    const real* drivers;
+    real t, h, a;
+
+    __device__ __forceinline__ void stage_state(real* Y) const {
+#pragma unroll
+        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
+    }
+    __device__ __forceinline__ void jvp(const real* v, real* out) const {
+        real Y[n];
+        stage_state(Y);
+        Sys::jvp(Y, params, drivers, v, out, t);
+    }
+    __device__ __forceinline__ void op(const real* v, real* out) const {
+        real jv[n];
+        jvp(v, jv);
+        const real ah = a * h;
+#pragma unroll
+        for (int i = 0; i < n; ++i) out[i] = v[i] - ah * jv[i];
+    }
+    __device__ __forceinline__ void precond(const real* v, real* out) const {
+        const real h_eff = h * a;
+#pragma unroll
+        for (int i = 0; i < n; ++i) out[i] = v[i];
+        for (int k = 0; k < Cfg::preconditioner_order; ++k) {
+            real jv[n];
+            jvp(out, jv);
+#pragma unroll
+            for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * jv[i];
+        }
+    }
+    __device__ __forceinline__ void residual(real* out) const {
+        real Y[n], f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1];
+        stage_state(Y);
+        Sys::dxdt(Y, params, drivers, obs, f, t);
+#pragma unroll
+        for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
+    }
+    __device__ __forceinline__ real krylov_norm2(const real* vals) const {
+        real nrm2 = real(0);
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            real tol = Cfg::krylov_atol[i] + Cfg::krylov_rtol[i] * dabs(base[i]);
+            tol = pmax(tol, real(1e-16));
+            const real r = dabs(vals[i]) / tol;
+            nrm2 += r * r;
+        }
+        return nrm2 * real(Cfg::inv_n);
+    }
+    __device__ __forceinline__ real correction_norm2(const real* vals) const {
+        real nrm2 = real(0);
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            const real sv = base[i] + a * u[i];
+            const real ref = pmax(dabs(sv), dabs(step_start[i]));
+            real tol = Cfg::newton_atol[i] + Cfg::newton_rtol[i] * ref;
+            tol = pmax(tol, real(1e-16));
+            const real r = vals[i] / tol;
+            nrm2 += r * r;
+        }
+        return nrm2 * real(Cfg::inv_n);
+    }
+};
+
+}  // namespace cubie

@@ -43,18 +43,29 @@ FAMILY_STEPPER = {
 
 @dataclass
 class ImplicitSettings:
-    """Newton-Krylov / linear-solver settings (reference defaults)."""
+    """Newton-Krylov / linear-solver settings (reference defaults).
 
-    newton_atol: float = 1e-7
-    newton_rtol: float = 1e-7
+    Tolerances are per-state arrays (broadcast from scalars)."""
+
+    newton_atol: object = 1e-7
+    newton_rtol: object = 1e-7
     newton_max_iters: int = 100
-    krylov_atol: float = 1e-6
-    krylov_rtol: float = 1e-6
+    krylov_atol: object = 1e-6
+    krylov_rtol: object = 1e-6
     krylov_max_iters: int = 100
     krylov_residual_reduction: Optional[float] = None
     krylov_residual_floor: Optional[float] = None
     linear_correction_type: str = "minimal_residual"
     preconditioner_order: int = 2
+    attempt_dense_prediction: bool = True
+
+    def tol(self, name: str, n: int) -> np.ndarray:
+        arr = np.atleast_1d(np.asarray(getattr(self, name), dtype=np.float64))
+        if arr.size == 1:
+            arr = np.full(n, float(arr[0]))
+        if arr.size != n:
+            raise ValueError(f"{name} must have length 1 or {n}")
+        return arr
 
 
 @dataclass
@@ -205,10 +216,34 @@ class KernelSpec:
         a(f"  static constexpr int krylov_max_iters = {int(im.krylov_max_iters)};")
         a(f"  static constexpr int linear_solver = {lin};")
         a(f"  static constexpr int preconditioner_order = {int(im.preconditioner_order)};")
-        a(f"  static constexpr real newton_atol = {self._lit(im.newton_atol)};")
-        a(f"  static constexpr real newton_rtol = {self._lit(im.newton_rtol)};")
-        a(f"  static constexpr real krylov_atol = {self._lit(im.krylov_atol)};")
-        a(f"  static constexpr real krylov_rtol = {self._lit(im.krylov_rtol)};")
+        for key in ("newton_atol", "newton_rtol", "krylov_atol", "krylov_rtol"):
+            a(f"  static constexpr real {key}[{max(n, 1)}] = {c_array(im.tol(key, n), p)};")
+        # dense stage prediction (DIRK / FIRK)
+        from .stage_predictors import dense_prediction_enabled, ratio_coefficients, source_stages
+        dense = (alg.family in ("dirk", "firk")
+                 and dense_prediction_enabled(tab, p, im.attempt_dense_prediction))
+        a(f"  static constexpr bool dense_prediction = {str(dense).lower()};")
+        if dense:
+            coeffs = ratio_coefficients(tab)
+            first = 0 if not tab.first_stage_is_explicit else 1
+            coeffs = coeffs[:, first:, :]
+            powers, rows_ = coeffs.shape[0], coeffs.shape[1]
+            flat = ", ".join(c_array(coeffs[k].reshape(-1), p) for k in range(powers))
+            a(f"  static constexpr int pred_powers = {powers};")
+            a(f"  static constexpr int pred_rows = {rows_};")
+            a(f"  static constexpr int pred_first = {first};")
+            a(f"  static constexpr real pred_coeffs[{powers}][{rows_ * st}] = {{{flat}}};")
+            a(f"  static constexpr real pred_max_ratio = {self._lit(tab.dense_prediction_ratio_limit(p))};")
+        else:
+            a("  static constexpr int pred_powers = 1;")
+            a("  static constexpr int pred_rows = 1;")
+            a("  static constexpr int pred_first = 0;")
+            a(f"  static constexpr real pred_coeffs[1][{st}] = {c_array([0.0] * st, p)};")
+            a("  static constexpr real pred_max_ratio = real(0);")
+        a(f"  static constexpr int pred_source[{st}] = {c_int_array(source_stages(tab.c))};")
+        diag = [tab.a[i][i] for i in range(st)]
+        later_explicit = not all(x != 0.0 for x in diag[1:])
+        a(f"  static constexpr bool dirk_has_later_explicit = {str(later_explicit).lower()};")
         red = im.krylov_residual_reduction if im.krylov_residual_reduction is not None else float(np.finfo(p).eps)
         flo = im.krylov_residual_floor if im.krylov_residual_floor is not None else float(np.sqrt(np.finfo(p).eps))
         a(f"  static constexpr real krylov_reduction = {self._lit(red)};")

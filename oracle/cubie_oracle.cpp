@@ -59,6 +59,7 @@ enum Metric {
     M_MEAN_STD_RMS = 9, M_MEAN_STD = 10, M_STD_RMS = 11,
     M_DXDT_MAX = 12, M_DXDT_MIN = 13, M_DXDT_EXTREMA = 14,
     M_D2XDT2_MAX = 15, M_D2XDT2_MIN = 16, M_D2XDT2_EXTREMA = 17,
+    M_SPIKE_COUNT = 18,   // engine extension: upward crossings of threshold `param`
 };
 
 static const int MAXS = 16;   // max stages
@@ -133,6 +134,19 @@ struct OracleConfig {
     jvp_cb_t jvp_cb;
     // controller powers through the deterministic pow (detmath.hpp)
     int32_t deterministic_pow;
+    // per-state inner tolerances (n entries each)
+    const double* krylov_atol_v;
+    const double* krylov_rtol_v;
+    const double* newton_atol_v;
+    const double* newton_rtol_v;
+    // dense stage prediction: coefficients [power][row][stage], rows from pred_first
+    int32_t dense_prediction;
+    int32_t pred_powers;
+    int32_t pred_first;
+    const double* pred_coeffs;
+    double pred_max_ratio;
+    // time derivative callback (SYS_CALLBACK, Rosenbrock-W); may be null
+    dfdt_cb_t dfdt_cb;
 };
 
 }  // extern "C"
@@ -151,7 +165,9 @@ struct DriverEval {
         T start_time = T(c->drv_t0);
         evaluation_start = T(start_time - (c->drv_pad_clamped ? resolution : T(0)));
     }
-    void eval(T time, T* out) const {
+    // derivative == true evaluates d/dt of each polynomial,
+    // array_interpolator.py:511-545
+    void eval(T time, T* out, bool derivative = false) const {
         const int nd = cfg->n_drivers;
         if (nd == 0) return;
         T scaled = (time - evaluation_start) * inv_resolution;
@@ -175,11 +191,15 @@ struct DriverEval {
         }
         const int order = cfg->drv_order;
         for (int i = 0; i < nd; ++i) {
+            const double* row = cfg->drv_coeffs + (size_t(seg) * nd + i) * (order + 1);
             T acc = T(0);
-            for (int k = order; k >= 0; --k) {
-                acc = acc * tau + T(cfg->drv_coeffs[(size_t(seg) * nd + i) * (order + 1) + k]);
+            if (derivative) {
+                for (int k = order; k > 0; --k) acc = acc * tau + T(k) * T(row[k]);
+                out[i] = in_range ? acc * inv_resolution : T(0);
+            } else {
+                for (int k = order; k >= 0; --k) acc = acc * tau + T(row[k]);
+                out[i] = in_range ? acc : T(0);
             }
-            out[i] = in_range ? acc : T(0);
         }
     }
 };
@@ -203,6 +223,10 @@ struct Problem {
         sys_jvp<T>(sys, y, p, d, v, jv, t);
     }
     void drivers(T t, T* out) const { drv.eval(t, out); }
+    void drivers_dt(T t, T* out) const { drv.eval(t, out, true); }
+    void dfdt(const T* y, const T* p, const T* d, const T* ddt, T* out, T t) const {
+        sys_dfdt<T>(sys, y, p, d, ddt, out, t);
+    }
 };
 
 // --------------------------------------------------------------------------
@@ -491,7 +515,7 @@ static int metric_buffer_size(int id, int param) {
     case M_PEAKS: case M_NEGATIVE_PEAKS: return 3 + param;
     case M_STD: case M_MEAN_STD_RMS: case M_MEAN_STD: case M_STD_RMS:
     case M_DXDT_EXTREMA: case M_D2XDT2_MAX: case M_D2XDT2_MIN: return 3;
-    case M_EXTREMA: case M_DXDT_MAX: case M_DXDT_MIN: return 2;
+    case M_EXTREMA: case M_DXDT_MAX: case M_DXDT_MIN: case M_SPIKE_COUNT: return 2;
     case M_D2XDT2_EXTREMA: return 4;
     default: return 1;
     }
@@ -572,6 +596,13 @@ static void metric_update(int id, int param, T value, T* buf, int32_t idx) {
         bool upd = (id == M_D2XDT2_MAX ? sd > buf[2] : sd < buf[2]) && idx >= 2;
         buf[2] = upd ? sd : buf[2];
         buf[1] = buf[0];
+        buf[0] = value;
+        break;
+    }
+    case M_SPIKE_COUNT: {
+        T thr = T(param);
+        bool cross = idx >= 1 && buf[0] < thr && value >= thr;
+        buf[1] = cross ? buf[1] + T(1) : buf[1];
         buf[0] = value;
         break;
     }
@@ -656,6 +687,7 @@ static void metric_save(int id, int param, T* buf, T* out, int32_t spp, T sample
         buf[2] = T(-1.0e30); buf[3] = T(1.0e30);
         break;
     }
+    case M_SPIKE_COUNT: out[0] = buf[1]; buf[1] = T(0); break;
     default: break;
     }
 }
@@ -926,6 +958,7 @@ static int solve_typed(const OracleConfig* cfg, int64_t n_runs, const double* in
     P.sys.rhs_cb = cfg->rhs_cb;
     P.sys.obs_cb = cfg->obs_cb;
     P.sys.jvp_cb = cfg->jvp_cb;
+    P.sys.dfdt_cb = cfg->dfdt_cb;
     P.drv.init(cfg);
     for (int64_t r = run_begin; r < run_end; ++r) {
         T init[MAXN], par[MAXN];

@@ -83,7 +83,63 @@ class OracleConfig(ctypes.Structure):
         ("krylov_residual_reduction", ctypes.c_double), ("krylov_floor", ctypes.c_double),
         ("rhs_cb", RHS_CB), ("obs_cb", OBS_CB), ("jvp_cb", JVP_CB),
         ("deterministic_pow", ctypes.c_int32),
+        ("krylov_atol_v", _D), ("krylov_rtol_v", _D), ("newton_atol_v", _D), ("newton_rtol_v", _D),
+        ("dense_prediction", ctypes.c_int32), ("pred_powers", ctypes.c_int32),
+        ("pred_first", ctypes.c_int32), ("pred_coeffs", _D), ("pred_max_ratio", ctypes.c_double),
     ]
+
+
+def _readahead(nodes, ratio, samples):
+    """Lagrange read-ahead of stage samples (reference stage_predictors.py:83-125)."""
+    s = len(nodes)
+    dn = [nodes[i] for i in samples]
+    m = np.zeros((s, s))
+    for ti in range(s):
+        target = 1.0 + ratio * nodes[ti]
+        for bi, bnode in enumerate(dn):
+            w = ratio
+            for oi, onode in enumerate(dn):
+                if oi != bi:
+                    w *= (target - onode) / (bnode - onode)
+            m[ti, samples[bi]] = w
+    return m
+
+
+def prediction_setup(tab, precision, attempt=True):
+    """Oracle-side restatement of the dense-prediction preconditions and ratio
+    polynomial coefficients (reference stage_predictors.py:127-300)."""
+    from numpy.polynomial import polynomial as npp
+    from numpy.polynomial.legendre import leggauss
+    nodes = [float(x) for x in tab.c]
+    last = {}
+    for st, nd in enumerate(nodes):
+        last[nd] = st
+    samples = tuple(last.values())
+    k = len(samples)
+    amp = np.abs(_readahead(nodes, 1.0, samples)).sum(axis=1).max()
+    g = list(0.5 * (leggauss(k)[0] + 1.0))
+    ref = np.abs(_readahead(g, 1.0, tuple(range(k)))).sum(axis=1).max()
+    limit = (tab.dense_prediction_ratio_float32 if precision is np.float32
+             else tab.dense_prediction_ratio_float64)
+    if not (attempt and limit > 0.0 and amp <= 10.0 * ref):
+        return None
+    dn = [nodes[i] for i in samples]
+    s = len(nodes)
+    out = np.zeros((k, s, s))
+    for bi in range(k):
+        others = [dn[j] for j in range(k) if j != bi]
+        coef = npp.polyfromroots(others) if others else np.array([1.0])
+        coef = coef / npp.polyval(dn[bi], coef)
+        # shift x -> 1 + x by composing with the polynomial 1 + x
+        sh = np.array([0.0])
+        for power, cval in enumerate(coef):
+            sh = npp.polyadd(sh, cval * npp.polypow([1.0, 1.0], power))
+        for p in range(k):
+            val = sh[p] if p < sh.size else 0.0
+            for ti in range(s):
+                out[p, ti, samples[bi]] = val * nodes[ti] ** p
+    first = 1 if tab.a[0][0] == 0.0 else 0
+    return out[:, first:, :], first, float(precision(limit))
 
 
 def build(force: bool = False) -> str:
@@ -233,8 +289,10 @@ class OracleRun:
         cfg.krylov_max_iters = im.krylov_max_iters
         cfg.linear_solver = LINEAR_IDS[im.linear_correction_type]
         cfg.preconditioner_order = im.preconditioner_order
-        cfg.newton_atol_s, cfg.newton_rtol_s = im.newton_atol, im.newton_rtol
-        cfg.krylov_atol_s, cfg.krylov_rtol_s = im.krylov_atol, im.krylov_rtol
+        cfg.newton_atol_s = float(np.min(im.newton_atol))
+        cfg.newton_rtol_s = float(np.min(im.newton_rtol))
+        cfg.krylov_atol_s = float(np.min(im.krylov_atol))
+        cfg.krylov_rtol_s = float(np.min(im.krylov_rtol))
         cfg.krylov_residual_reduction = (im.krylov_residual_reduction
                                          if im.krylov_residual_reduction is not None
                                          else float(np.finfo(prec).eps))
@@ -248,6 +306,27 @@ class OracleRun:
             cfg.obs_cb = OBS_CB()
             cfg.jvp_cb = JVP_CB()
         cfg.deterministic_pow = int(getattr(k, "deterministic_pow", False))
+        n = sysm.num_states
+        cfg.krylov_atol_v = _dp(arr(im.tol("krylov_atol", n)))
+        cfg.krylov_rtol_v = _dp(arr(im.tol("krylov_rtol", n)))
+        cfg.newton_atol_v = _dp(arr(im.tol("newton_atol", n)))
+        cfg.newton_rtol_v = _dp(arr(im.tol("newton_rtol", n)))
+        pred = None
+        if alg.family in ("dirk", "firk"):
+            pred = prediction_setup(tab, prec, im.attempt_dense_prediction)
+        if pred is not None:
+            coeffs, first, ratio = pred
+            cfg.dense_prediction = 1
+            cfg.pred_powers = coeffs.shape[0]
+            cfg.pred_first = first
+            cfg.pred_coeffs = _dp(arr(coeffs.reshape(-1)))
+            cfg.pred_max_ratio = ratio
+        else:
+            cfg.dense_prediction = 0
+            cfg.pred_powers = 1
+            cfg.pred_first = 0
+            cfg.pred_coeffs = _dp(arr([0.0]))
+            cfg.pred_max_ratio = 0.0
         self.cfg = cfg
         self.keep = keep
         self.kernel = k

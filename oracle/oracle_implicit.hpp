@@ -1,7 +1,463 @@
 // Implicit step families for the oracle (test infrastructure only).
+//
+// Restates, on the CPU:
+//   DIRK step ....... integrators/algorithms/generic_dirk.py:515-857
+//   dense predictor . integrators/stage_predictors.py:345-548
+//   Newton-Krylov ... integrators/matrix_free_solvers/newton_krylov.py:276-518
+//   MR / SD ......... matrix_free_solvers/linear_solver.py:157-563
+//   BiCGSTAB ........ matrix_free_solvers/bicgstab_solver.py:226-672
+//   norms ........... integrators/norms.py:125-432
+//   operator / residual / Neumann preconditioner ..
+//                     odesystems/symbolic/codegen/linear_operators.py:118-298,
+//                     nonlinear_residuals.py:59-87, preconditioners.py:85-307
+// (beta = gamma = 1, identity mass matrix), and the FIRK and Rosenbrock-W
+// steps below.
 #pragma once
 
 namespace oracle {
+
+template <typename T> struct SC;
+template <> struct SC<double> {
+    static constexpr double clamp = 1e150, breakdown = 1e-200;
+    static constexpr double huge = 1.7976931348623157e308;
+};
+template <> struct SC<float> {
+    static constexpr float clamp = 1e16f, breakdown = 1e-30f;
+    static constexpr float huge = 3.4028234663852886e38f;
+};
+template <typename T> static inline T clampc(T x) {
+    const T c = SC<T>::clamp;
+    if (x > c) x = c;
+    if (x < -c) x = -c;
+    return x;
+}
+template <typename T> static inline T clamp_pos(T x) { return x > SC<T>::clamp ? SC<T>::clamp : x; }
+
+// Stage problem for one implicit stage of width n: u = h f(t, base + a u).
+template <typename T>
+struct StageProblem {
+    const Problem<T>* P;
+    const OracleConfig* cfg;
+    int n;
+    T* u;
+    const T* base;
+    const T* step_start;
+    const T* params;
+    const T* drivers;
+    T t, h, a;
+    T katol[MAXN], krtol[MAXN], natol[MAXN], nrtol[MAXN];
+    T inv_n;
+
+    void jvp(const T* v, T* out) const {
+        T Y[MAXN];
+        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
+        P->jvp(Y, params, drivers, v, out, t);
+    }
+    void op(const T* v, T* out) const {
+        T jv[MAXN];
+        jvp(v, jv);
+        T ah = a * h;
+        for (int i = 0; i < n; ++i) out[i] = v[i] - ah * jv[i];
+    }
+    void precond(const T* v, T* out) const {
+        T h_eff = h * a;
+        for (int i = 0; i < n; ++i) out[i] = v[i];
+        for (int k = 0; k < cfg->preconditioner_order; ++k) {
+            T jv[MAXN];
+            jvp(out, jv);
+            for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * jv[i];
+        }
+    }
+    void residual(T* out) const {
+        T Y[MAXN], f[MAXN], obs[MAXN];
+        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
+        P->rhs(Y, params, drivers, obs, f, t);
+        for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
+    }
+    T krylov_norm2(const T* v) const {
+        T s = T(0);
+        for (int i = 0; i < n; ++i) {
+            T tol = katol[i] + krtol[i] * std::fabs(base[i]);
+            tol = std::max(tol, T(1e-16));
+            T r = std::fabs(v[i]) / tol;
+            s += r * r;
+        }
+        return s * inv_n;
+    }
+    T correction_norm2(const T* v) const {
+        T s = T(0);
+        for (int i = 0; i < n; ++i) {
+            T sv = base[i] + a * u[i];
+            T ref = std::max(std::fabs(sv), std::fabs(step_start[i]));
+            T tol = natol[i] + nrtol[i] * ref;
+            tol = std::max(tol, T(1e-16));
+            T r = v[i] / tol;
+            s += r * r;
+        }
+        return s * inv_n;
+    }
+};
+
+template <typename T, typename Prob>
+static int32_t mr_solve(const OracleConfig* cfg, Prob& Pb, int W, T* rhs, T* x, bool x_zero, int& iters) {
+    const T red = T(cfg->krylov_residual_reduction), flo = T(cfg->krylov_floor);
+    T rn2 = Pb.krylov_norm2(rhs);
+    T tol = flo + red * T(std::sqrt(rn2));
+    T tol2 = tol * tol;
+    T temp[4 * MAXN], z[4 * MAXN];
+    if (!x_zero) {
+        Pb.op(x, temp);
+        for (int i = 0; i < W; ++i) rhs[i] = rhs[i] - temp[i];
+    }
+    T acc = Pb.krylov_norm2(rhs);
+    bool conv = acc <= tol2;
+    int it = 0;
+    const bool sd = cfg->linear_solver == 1;
+    for (int k = 0; k < cfg->krylov_max_iters; ++k) {
+        if (conv) break;
+        ++it;
+        Pb.precond(rhs, z);
+        Pb.op(z, temp);
+        T num = T(0), den = T(0);
+        for (int i = 0; i < W; ++i) {
+            if (sd) { num += rhs[i] * z[i]; den += temp[i] * z[i]; }
+            else { num += temp[i] * rhs[i]; den += temp[i] * temp[i]; }
+        }
+        T alpha = den != T(0) ? num / den : T(0);
+        for (int i = 0; i < W; ++i) { x[i] += alpha * z[i]; rhs[i] -= alpha * temp[i]; }
+        acc = Pb.krylov_norm2(rhs);
+        conv = conv || acc <= tol2;
+    }
+    iters = it;
+    return conv ? ST_SUCCESS : ST_LINEAR_MAX_ITERS;
+}
+
+template <typename T, typename Prob>
+static int32_t bicgstab_solve(const OracleConfig* cfg, Prob& Pb, int W, T* rhs, T* x, bool x_zero, int& iters) {
+    const T red = T(cfg->krylov_residual_reduction), flo = T(cfg->krylov_floor);
+    const T dclamp = SC<T>::clamp, brk = SC<T>::breakdown;
+    T rn2 = Pb.krylov_norm2(rhs);
+    T tol = flo + red * T(std::sqrt(rn2));
+    T tol2 = tol * tol;
+    T r0[4 * MAXN], p[4 * MAXN], v[4 * MAXN], tmp[4 * MAXN], sh[4 * MAXN];
+    if (!x_zero) Pb.op(x, tmp);
+    else for (int i = 0; i < W; ++i) tmp[i] = T(0);
+    T rho_prev = T(0);
+    for (int i = 0; i < W; ++i) {
+        T ax = clampc(tmp[i]);
+        T ri = rhs[i] - ax;
+        rhs[i] = ri;
+        r0[i] = ri;
+        p[i] = clampc(ri);
+        rho_prev += clamp_pos(ri * ri);
+    }
+    T acc = Pb.krylov_norm2(rhs);
+    bool conv = acc <= tol2, broken = false, fin = conv;
+    int it = 0;
+    for (int k = 0; k < cfg->krylov_max_iters; ++k) {
+        if (fin) break;
+        ++it;
+        Pb.precond(p, tmp);
+        for (int i = 0; i < W; ++i) tmp[i] = clampc(tmp[i]);
+        Pb.op(tmp, v);
+        T d_r0v = T(0);
+        for (int i = 0; i < W; ++i) { v[i] = clampc(v[i]); d_r0v += clampc(r0[i] * v[i]); }
+        bool a_over = std::fabs(rho_prev) > std::fabs(d_r0v) * dclamp;
+        broken = broken || (!fin && a_over);
+        fin = conv || broken;
+        T alpha = (d_r0v != T(0) && !a_over) ? rho_prev / d_r0v : T(0);
+        T ae = fin ? T(0) : alpha;
+        for (int i = 0; i < W; ++i) { x[i] = x[i] + ae * tmp[i]; rhs[i] = rhs[i] - ae * v[i]; }
+        acc = Pb.krylov_norm2(rhs);
+        conv = conv || acc <= tol2;
+        fin = conv || broken;
+        Pb.precond(rhs, sh);
+        for (int i = 0; i < W; ++i) sh[i] = clampc(sh[i]);
+        Pb.op(sh, tmp);
+        T dts = T(0), dtt = T(0);
+        for (int i = 0; i < W; ++i) {
+            T ti = clampc(tmp[i]);
+            tmp[i] = ti;
+            dts += clampc(ti * rhs[i]);
+            dtt += clamp_pos(ti * ti);
+        }
+        bool o_over = std::fabs(dts) > dtt * dclamp;
+        T omega = (dtt != T(0) && !o_over) ? dts / dtt : T(0);
+        T oe = fin ? T(0) : omega;
+        for (int i = 0; i < W; ++i) { x[i] = x[i] + oe * sh[i]; rhs[i] = rhs[i] - oe * tmp[i]; }
+        acc = Pb.krylov_norm2(rhs);
+        conv = conv || acc <= tol2;
+        T rho_new = T(0);
+        for (int i = 0; i < W; ++i) rho_new += clampc(r0[i] * rhs[i]);
+        bool rho_bad = std::fabs(rho_new) < brk, om_bad = std::fabs(omega) < brk;
+        broken = broken || (!conv && (rho_bad || om_bad));
+        fin = conv || broken;
+        bool b_over = (std::fabs(rho_new) > std::fabs(rho_prev) * dclamp) ||
+                      (std::fabs(alpha) > std::fabs(omega) * dclamp);
+        broken = broken || (!fin && b_over);
+        fin = conv || broken;
+        T beta = !fin ? (rho_new / rho_prev) * (alpha / omega) : T(0);
+        for (int i = 0; i < W; ++i) {
+            T pi = !fin ? rhs[i] + beta * (p[i] - omega * v[i]) : p[i];
+            p[i] = clampc(pi);
+        }
+        rho_prev = !fin ? rho_new : rho_prev;
+    }
+    iters = it;
+    return conv ? ST_SUCCESS : (broken ? ST_BICGSTAB_BREAKDOWN : ST_LINEAR_MAX_ITERS);
+}
+
+template <typename T, typename Prob>
+static int32_t linear_solve(const OracleConfig* cfg, Prob& Pb, int W, T* rhs, T* x, bool x_zero, int& it) {
+    if (cfg->linear_solver == 2) return bicgstab_solve<T>(cfg, Pb, W, rhs, x, x_zero, it);
+    return mr_solve<T>(cfg, Pb, W, rhs, x, x_zero, it);
+}
+
+template <typename T, typename Prob>
+static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_store, int32_t* counters) {
+    const T kappa = T(0.01), first_bound = T(1.0e-5), decay = T(0.3), div = T(2.0);
+    const T huge = SC<T>::huge;
+    const T stag = T(100.0 * std::sqrt(double(std::numeric_limits<T>::epsilon())));
+    T prev_theta = theta_store > T(0) ? theta_store : T(1);
+    T ndz_prev = T(0);
+    bool conv = false, failed = false;
+    int32_t last_lin = ST_SUCCESS;
+    int iters = 0, ktot = 0, iteration = 0;
+    T res[4 * MAXN], delta[4 * MAXN];
+    for (int k = 0; k < cfg->newton_max_iters; ++k) {
+        if (conv || failed) break;
+        ++iteration;
+        Pb.residual(res);
+        for (int i = 0; i < W; ++i) { res[i] = -res[i]; delta[i] = T(0); }
+        int kit = 0;
+        int32_t lin = linear_solve<T>(cfg, Pb, W, res, delta, true, kit);
+        ktot += kit;
+        last_lin = lin;
+        ++iters;
+        T n2 = Pb.correction_norm2(delta);
+        T ndz = T(std::sqrt(n2));
+        bool judged = lin == ST_SUCCESS;
+        bool hist = ndz_prev > T(0);
+        T nps = hist ? ndz_prev : T(1);
+        T theta = hist ? std::max(decay * prev_theta, ndz / nps) : prev_theta;
+        bool small_first = iteration == 1 && ndz < first_bound;
+        bool eta_ok = theta < T(1) && theta * ndz < kappa * (T(1) - theta);
+        bool nonfinite = !(n2 <= huge);
+        bool stagnant = judged && hist && std::fabs(theta - T(1)) <= stag;
+        bool diverging = judged && ((hist && theta > div) || nonfinite);
+        bool conv_stag = stagnant && ndz <= T(1) && !diverging;
+        bool failed_now = diverging || (stagnant && ndz > T(1));
+        failed = failed || failed_now;
+        bool commit = judged && !failed_now && !conv_stag;
+        for (int i = 0; i < W; ++i) Pb.u[i] = commit ? Pb.u[i] + delta[i] : Pb.u[i];
+        conv = conv || conv_stag || (commit && (eta_ok || small_first));
+        ndz_prev = commit ? ndz : T(0);
+        prev_theta = (judged && hist) ? theta : prev_theta;
+    }
+    theta_store = conv ? prev_theta : T(1);
+    int32_t fb = failed ? ST_NEWTON_DIVERGENCE : ST_NEWTON_MAX_ITERS;
+    if (last_lin != ST_SUCCESS) fb |= last_lin;
+    counters[0] = iters;
+    counters[1] = ktot;
+    return conv ? ST_SUCCESS : fb;
+}
+
+template <typename T>
+static void fill_inner_tols(const OracleConfig* cfg, int n, T* katol, T* krtol, T* natol, T* nrtol) {
+    for (int i = 0; i < n; ++i) {
+        katol[i] = T(cfg->krylov_atol_v[i]);
+        krtol[i] = T(cfg->krylov_rtol_v[i]);
+        natol[i] = T(cfg->newton_atol_v[i]);
+        nrtol[i] = T(cfg->newton_rtol_v[i]);
+    }
+}
+
+// --------------------------------------------------------------------------
+// DIRK step
+// --------------------------------------------------------------------------
+template <typename T>
+struct DIRKStepper {
+    const Problem<T>* P;
+    const OracleConfig* cfg;
+    int s, n;
+    T A[MAXS][MAXS], b[MAXS], c[MAXS], d[MAXS];
+    bool has_error, fsal, accumulates_output, accumulates_error;
+    int b_row, b_hat_row;
+    bool dense;
+    T stage_rhs[MAXN], stage_increment[MAXN], history[MAXS * MAXN];
+    T previous_step_size, theta;
+    T katol[MAXN], krtol[MAXN], natol[MAXN], nrtol[MAXN];
+
+    void init(const Problem<T>* p) {
+        P = p;
+        cfg = p->cfg;
+        s = cfg->stages;
+        n = p->n;
+        for (int i = 0; i < s; ++i) {
+            for (int j = 0; j < s; ++j) A[i][j] = T(cfg->a[i * s + j]);
+            b[i] = T(cfg->b[i]);
+            c[i] = T(cfg->c[i]);
+        }
+        has_error = cfg->has_error != 0;
+        for (int i = 0; i < s; ++i) d[i] = has_error ? T(cfg->b[i] - cfg->b_hat[i]) : T(0);
+        bool a_last_eq_b = true, first_row_zero = true;
+        for (int i = 0; i < s; ++i) {
+            if (cfg->a[(s - 1) * s + i] != cfg->b[i]) a_last_eq_b = false;
+            if (cfg->a[i] != 0.0) first_row_zero = false;
+        }
+        fsal = cfg->c[0] == 0.0 && cfg->c[s - 1] == 1.0 && a_last_eq_b && first_row_zero;
+        b_row = ERKStepper<T>::find_row(cfg, cfg->b);
+        b_hat_row = has_error ? ERKStepper<T>::find_row(cfg, cfg->b_hat) : -1;
+        accumulates_output = b_row < 0;
+        accumulates_error = b_hat_row < 0;
+        dense = cfg->dense_prediction != 0;
+        for (int i = 0; i < MAXN; ++i) { stage_rhs[i] = T(0); stage_increment[i] = T(0); }
+        for (int i = 0; i < MAXS * MAXN; ++i) history[i] = T(0);
+        previous_step_size = T(0);
+        theta = T(0);
+        fill_inner_tols<T>(cfg, n, katol, krtol, natol, nrtol);
+    }
+
+    void predict(T ratio, bool apply) {
+        const int Pp = cfg->pred_powers, F = cfg->pred_first, R = s - F;
+        T tr[MAXS * MAXS];
+        for (int e = 0; e < R * s; ++e) {
+            T acc = T(0);
+            for (int p = Pp - 1; p >= 0; --p) acc = acc * ratio + T(cfg->pred_coeffs[p * R * s + e]);
+            tr[e] = acc * ratio;
+        }
+        for (int i = 0; i < n; ++i) {
+            T prev[MAXS];
+            for (int k = 0; k < s; ++k) prev[k] = history[k * n + i];
+            for (int r = 0; r < R; ++r) {
+                T acc = T(0);
+                for (int k = 0; k < s; ++k) acc += tr[r * s + k] * prev[k];
+                int target = (F + r) * n + i;
+                history[target] = apply ? acc : history[target];
+            }
+        }
+    }
+
+    int32_t solve(const T* base, const T* state, const T* params, const T* drivers, T t, T h, T a,
+                  int32_t* counters) {
+        StageProblem<T> sp;
+        sp.P = P; sp.cfg = cfg; sp.n = n; sp.u = stage_increment; sp.base = base;
+        sp.step_start = state; sp.params = params; sp.drivers = drivers;
+        sp.t = t; sp.h = h; sp.a = a;
+        for (int i = 0; i < n; ++i) {
+            sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
+        }
+        sp.inv_n = T(1.0 / n);
+        return newton_solve<T>(cfg, sp, n, theta, counters);
+    }
+
+    int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,
+                 T* observables, T* prop_obs, T* error, T dt, T time, bool first_step,
+                 bool accepted_flag, int32_t* counters) {
+        T current_time = time;
+        T end_time = current_time + dt;
+        T acc[MAXS * MAXN], stage_base[MAXN];
+        int32_t status = ST_SUCCESS;
+        for (int i = 0; i < (s - 1) * n; ++i) acc[i] = T(0);
+        for (int i = 0; i < n; ++i) if (has_error && accumulates_error) error[i] = T(0);
+        if (dense) {
+            T prev_dt = previous_step_size;
+            previous_step_size = dt;
+            T safe = prev_dt > T(0) ? prev_dt : dt;
+            T ratio = dt / safe;
+            bool apply = !first_step && accepted_flag && ratio <= T(cfg->pred_max_ratio);
+            predict(ratio, apply);
+        }
+        bool use_cached = false;
+        if (fsal && s > 1 && !first_step) use_cached = accepted_flag;
+        T stage_time = current_time + dt * c[0];
+        for (int i = 0; i < n; ++i) {
+            stage_base[i] = state[i];
+            if (accumulates_output) proposed[i] = T(0);
+        }
+        if (!use_cached) {
+            if (cfg->c[0] == 0.0) {
+                for (int i = 0; i < cfg->n_drivers; ++i) prop_drivers[i] = drivers[i];
+            } else {
+                P->drivers(stage_time, prop_drivers);
+            }
+            if (A[0][0] != T(0)) {
+                if (dense) for (int i = 0; i < n; ++i) stage_increment[i] = history[i];
+                status |= solve(stage_base, state, params, prop_drivers, stage_time, dt, A[0][0], counters);
+                if (dense) for (int i = 0; i < n; ++i) history[i] = stage_increment[i];
+                for (int i = 0; i < n; ++i) stage_base[i] += A[0][0] * stage_increment[i];
+            }
+            P->observables(stage_base, params, prop_drivers, prop_obs, stage_time);
+            P->rhs(stage_base, params, prop_drivers, prop_obs, stage_rhs, stage_time);
+        }
+        if (dense && A[0][0] == T(0)) for (int i = 0; i < n; ++i) history[i] = dt * stage_rhs[i];
+        for (int i = 0; i < n; ++i) {
+            T rv = stage_rhs[i];
+            if (accumulates_output) proposed[i] += b[0] * rv;
+            else if (b_row == 0) proposed[i] = stage_base[i];
+            if (has_error) {
+                if (accumulates_error) error[i] += d[0] * rv;
+                else if (b_hat_row == 0) error[i] = stage_base[i];
+            }
+        }
+        bool later_explicit = false;
+        for (int k = 1; k < s; ++k) if (A[k][k] == T(0)) later_explicit = true;
+        for (int prev = 0; prev < s - 1; ++prev) {
+            int si = prev + 1;
+            for (int succ = 0; succ < s - 1; ++succ) {
+                T coeff = (succ + 1 > prev) ? A[succ + 1][prev] : T(0);
+                for (int i = 0; i < n; ++i) acc[succ * n + i] += coeff * stage_rhs[i];
+            }
+            stage_time = current_time + dt * c[si];
+            P->drivers(stage_time, prop_drivers);
+            for (int i = 0; i < n; ++i) stage_base[i] = acc[prev * n + i] * dt + state[i];
+            T diag = A[si][si];
+            if (diag != T(0)) {
+                if (dense) {
+                    int src = source_stage(si);
+                    for (int i = 0; i < n; ++i) stage_increment[i] = history[src * n + i];
+                }
+                status |= solve(stage_base, state, params, prop_drivers, stage_time, dt, diag, counters);
+                if (dense) for (int i = 0; i < n; ++i) history[si * n + i] = stage_increment[i];
+                for (int i = 0; i < n; ++i) stage_base[i] += diag * stage_increment[i];
+            }
+            P->observables(stage_base, params, prop_drivers, prop_obs, stage_time);
+            P->rhs(stage_base, params, prop_drivers, prop_obs, stage_rhs, stage_time);
+            if (dense && later_explicit && diag == T(0))
+                for (int i = 0; i < n; ++i) history[si * n + i] = dt * stage_rhs[i];
+            for (int i = 0; i < n; ++i) {
+                T inc = stage_rhs[i];
+                if (accumulates_output) proposed[i] += b[si] * inc;
+                else if (b_row == si) proposed[i] = stage_base[i];
+                if (has_error) {
+                    if (accumulates_error) error[i] += d[si] * inc;
+                    else if (b_hat_row == si) error[i] = stage_base[i];
+                }
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            if (accumulates_output) { proposed[i] *= dt; proposed[i] += state[i]; }
+            if (has_error) {
+                if (accumulates_error) error[i] *= dt;
+                else error[i] = proposed[i] - error[i];
+            }
+        }
+        P->drivers(end_time, prop_drivers);
+        P->observables(proposed, params, prop_drivers, prop_obs, end_time);
+        return status;
+    }
+
+    // generic_dirk_tableaus.py prediction_source_stages: the earliest stage
+    // sharing this stage's node supplies its starting guess.
+    int source_stage(int si) const {
+        int src = si;
+        for (int k = 0; k < si; ++k) {
+            if (cfg->c[k] == cfg->c[si]) { src = k; }
+        }
+        // latest earlier stage at the same node
+        return src;
+    }
+};
 
 template <typename T>
 struct UnsupportedStepper {
@@ -11,8 +467,417 @@ struct UnsupportedStepper {
         return -1;
     }
 };
-template <typename T> using DIRKStepper = UnsupportedStepper<T>;
-template <typename T> using FIRKStepper = UnsupportedStepper<T>;
-template <typename T> using RosenbrockStepper = UnsupportedStepper<T>;
+// --------------------------------------------------------------------------
+// FIRK step: all s stages solved as one coupled Newton problem of width s*n.
+//   step ............. integrators/algorithms/generic_firk.py:527-704
+//   residual ......... codegen/nonlinear_residuals.py:193-362
+//   operator ......... codegen/linear_operators.py:599-755
+//   Neumann .......... codegen/preconditioners.py:200-255, 344-399
+//   norms ............ integrators/norms.py:275-324 (tiled Krylov norm),
+//                      :377-432 (FIRK correction norm)
+// Stage i of the unknown u holds z_i = Y_i - y_n, with Y_i = y_n + sum_j a_ij z_j
+// (the tableau's a, not the a_ij argument, which the coupled solve passes as 0).
+// --------------------------------------------------------------------------
+static const int MAXW = 4 * MAXN;   // widest coupled vector the solvers hold
+
+template <typename T>
+struct FIRKProblem {
+    const Problem<T>* P;
+    const OracleConfig* cfg;
+    int s, n, nd;
+    const T (*A)[MAXS];
+    const T* c;
+    T* u;
+    const T* base;
+    const T* step_start;
+    const T* params;
+    const T* drivers;   // s stacked driver vectors, one per stage time
+    T t, h;
+    const T *katol, *krtol, *natol, *nrtol;   // s*n entries
+    T inv_w;
+
+    void combine(int i, const T* vec, T* out, bool with_base) const {
+        for (int k = 0; k < n; ++k) {
+            T acc = with_base ? base[k] : T(0);
+            for (int j = 0; j < s; ++j)
+                if (A[i][j] != T(0)) acc += A[i][j] * vec[j * n + k];
+            out[k] = acc;
+        }
+    }
+    T stage_time(int i) const { return t + h * c[i]; }
+    const T* stage_drivers(int i) const { return drivers + i * nd; }
+
+    void op(const T* v, T* out) const {
+        for (int i = 0; i < s; ++i) {
+            T Y[MAXN], dir[MAXN], jv[MAXN];
+            combine(i, u, Y, true);
+            combine(i, v, dir, false);
+            P->jvp(Y, params, stage_drivers(i), dir, jv, stage_time(i));
+            for (int k = 0; k < n; ++k) out[i * n + k] = v[i * n + k] - h * jv[k];
+        }
+    }
+    void precond(const T* v, T* out) const {
+        const int W = s * n;
+        for (int e = 0; e < W; ++e) out[e] = v[e];
+        for (int k = 0; k < cfg->preconditioner_order; ++k) {
+            T jvp[MAXW];
+            for (int i = 0; i < s; ++i) {
+                T Y[MAXN], dir[MAXN];
+                combine(i, u, Y, true);
+                combine(i, out, dir, false);
+                P->jvp(Y, params, stage_drivers(i), dir, jvp + i * n, stage_time(i));
+            }
+            for (int e = 0; e < W; ++e) out[e] = v[e] + h * jvp[e];
+        }
+    }
+    void residual(T* out) const {
+        for (int i = 0; i < s; ++i) {
+            T Y[MAXN], f[MAXN], obs[MAXN];
+            combine(i, u, Y, true);
+            P->rhs(Y, params, stage_drivers(i), obs, f, stage_time(i));
+            for (int k = 0; k < n; ++k) out[i * n + k] = u[i * n + k] - h * f[k];
+        }
+    }
+    T krylov_norm2(const T* v) const {
+        T acc = T(0);
+        for (int e = 0; e < s * n; ++e) {
+            T tol = katol[e] + krtol[e] * std::fabs(base[e % n]);
+            tol = std::max(tol, T(1e-16));
+            T r = std::fabs(v[e]) / tol;
+            acc += r * r;
+        }
+        return acc * inv_w;
+    }
+    T correction_norm2(const T* v) const {
+        T acc = T(0);
+        for (int e = 0; e < s * n; ++e) {
+            const int i = e / n, k = e % n;
+            T sv = base[k];
+            for (int j = 0; j < s; ++j) sv += A[i][j] * u[j * n + k];
+            T ref = std::max(std::fabs(sv), std::fabs(step_start[k]));
+            T tol = natol[e] + nrtol[e] * ref;
+            tol = std::max(tol, T(1e-16));
+            T r = v[e] / tol;
+            acc += r * r;
+        }
+        return acc * inv_w;
+    }
+};
+
+template <typename T>
+struct FIRKStepper {
+    const Problem<T>* P;
+    const OracleConfig* cfg;
+    int s, n, nd;
+    T A[MAXS][MAXS], b[MAXS], c[MAXS], d[MAXS];
+    bool has_error, accumulates_output, accumulates_error, ends_at_one, dense;
+    int b_row, b_hat_row;
+    T stage_increment[MAXW];
+    T previous_step_size, theta;
+    T katol[MAXW], krtol[MAXW], natol[MAXW], nrtol[MAXW];
+    T stage_drivers[MAXS * 16];
+
+    void init(const Problem<T>* p) {
+        P = p;
+        cfg = p->cfg;
+        s = cfg->stages;
+        n = p->n;
+        nd = cfg->n_drivers;
+        for (int i = 0; i < s; ++i) {
+            for (int j = 0; j < s; ++j) A[i][j] = T(cfg->a[i * s + j]);
+            b[i] = T(cfg->b[i]);
+            c[i] = T(cfg->c[i]);
+        }
+        has_error = cfg->has_error != 0;
+        for (int i = 0; i < s; ++i) d[i] = has_error ? T(cfg->b[i] - cfg->b_hat[i]) : T(0);
+        b_row = ERKStepper<T>::find_row(cfg, cfg->b);
+        b_hat_row = has_error ? ERKStepper<T>::find_row(cfg, cfg->b_hat) : -1;
+        accumulates_output = b_row < 0;
+        accumulates_error = b_hat_row < 0;
+        ends_at_one = c[s - 1] == T(1);
+        dense = cfg->dense_prediction != 0;
+        for (int e = 0; e < MAXW; ++e) stage_increment[e] = T(0);
+        previous_step_size = T(0);
+        theta = T(0);
+        // The coupled solve weighs entry e with the tolerance of state e % n.
+        for (int e = 0; e < s * n; ++e) {
+            katol[e] = T(cfg->krylov_atol_v[e % n]);
+            krtol[e] = T(cfg->krylov_rtol_v[e % n]);
+            natol[e] = T(cfg->newton_atol_v[e % n]);
+            nrtol[e] = T(cfg->newton_rtol_v[e % n]);
+        }
+    }
+
+    // stage_predictors.py:492-548 applied to the coupled increment in place.
+    void predict(T ratio, bool apply) {
+        const int Pp = cfg->pred_powers, F = cfg->pred_first, R = s - F;
+        T tr[MAXS * MAXS];
+        for (int e = 0; e < R * s; ++e) {
+            T acc = T(0);
+            for (int q = Pp - 1; q >= 0; --q) acc = acc * ratio + T(cfg->pred_coeffs[q * R * s + e]);
+            tr[e] = acc * ratio;
+        }
+        for (int k = 0; k < n; ++k) {
+            T prev[MAXS];
+            for (int j = 0; j < s; ++j) prev[j] = stage_increment[j * n + k];
+            for (int r = 0; r < R; ++r) {
+                T acc = T(0);
+                for (int j = 0; j < s; ++j) acc += tr[r * s + j] * prev[j];
+                const int target = (F + r) * n + k;
+                stage_increment[target] = apply ? acc : stage_increment[target];
+            }
+        }
+    }
+
+    int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,
+                 T* observables, T* prop_obs, T* error, T dt, T time, bool first_step,
+                 bool accepted_flag, int32_t* counters) {
+        (void)drivers;
+        (void)observables;
+        const T current_time = time;
+        const T end_time = current_time + dt;
+        int32_t status = ST_SUCCESS;
+        if (dense) {
+            T prev_dt = previous_step_size;
+            previous_step_size = dt;
+            T safe = prev_dt > T(0) ? prev_dt : dt;
+            T ratio = dt / safe;
+            bool apply = !first_step && accepted_flag && ratio <= T(cfg->pred_max_ratio);
+            predict(ratio, apply);
+        }
+        for (int k = 0; k < n; ++k) {
+            if (accumulates_output) proposed[k] = state[k];
+            if (has_error && accumulates_error) error[k] = T(0);
+        }
+        if (nd > 0)
+            for (int i = 0; i < s; ++i) P->drivers(current_time + dt * c[i], stage_drivers + i * nd);
+
+        FIRKProblem<T> fp;
+        fp.P = P; fp.cfg = cfg; fp.s = s; fp.n = n; fp.nd = nd;
+        fp.A = A; fp.c = c; fp.u = stage_increment; fp.base = state; fp.step_start = state;
+        fp.params = params; fp.drivers = stage_drivers; fp.t = current_time; fp.h = dt;
+        fp.katol = katol; fp.krtol = krtol; fp.natol = natol; fp.nrtol = nrtol;
+        fp.inv_w = T(1.0 / double(s * n));
+        status |= newton_solve<T>(cfg, fp, s * n, theta, counters);
+
+        for (int i = 0; i < s; ++i) {
+            for (int q = 0; q < nd; ++q) prop_drivers[q] = stage_drivers[i * nd + q];
+            T Y[MAXN];
+            fp.combine(i, stage_increment, Y, true);
+            if (!accumulates_output && b_row == i)
+                for (int k = 0; k < n; ++k) proposed[k] = Y[k];
+            if (!accumulates_error && b_hat_row == i)
+                for (int k = 0; k < n; ++k) error[k] = Y[k];
+        }
+        // Compensated (Kahan) weighted sums of the stage increments.
+        if (accumulates_output) {
+            for (int k = 0; k < n; ++k) {
+                T acc = T(0), comp = T(0);
+                for (int i = 0; i < s; ++i) {
+                    T term = b[i] * stage_increment[i * n + k] - comp;
+                    T next = acc + term;
+                    comp = (next - acc) - term;
+                    acc = next;
+                }
+                proposed[k] = state[k] + acc;
+            }
+        }
+        if (has_error && accumulates_error) {
+            for (int k = 0; k < n; ++k) {
+                T acc = T(0), comp = T(0);
+                for (int i = 0; i < s; ++i) {
+                    T term = d[i] * stage_increment[i * n + k] - comp;
+                    T next = acc + term;
+                    comp = (next - acc) - term;
+                    acc = next;
+                }
+                error[k] = acc;
+            }
+        }
+        if (!ends_at_one && nd > 0) P->drivers(end_time, prop_drivers);
+        P->observables(proposed, params, prop_drivers, prop_obs, end_time);
+        if (!accumulates_error)
+            for (int k = 0; k < n; ++k) error[k] = proposed[k] - error[k];
+        return status;
+    }
+};
+
+// --------------------------------------------------------------------------
+// Rosenbrock-W step: one linear solve (I - gamma h J) K_i = rhs_i per stage
+// with J taken at the step start.
+//   step ............. integrators/algorithms/generic_rosenbrock_w.py:470-740
+//   operator ......... codegen/linear_operators.py:256-335 (cached form:
+//                      beta v - gamma a_ij h J v with a_ij = 1, beta = 1)
+//   Neumann .......... codegen/preconditioners.py:143-198 (h_eff = h gamma)
+//   linear solvers ... linear_solver.py:225-400, bicgstab_solver.py (norm
+//                      referenced to the step-start state, initial guess x)
+// --------------------------------------------------------------------------
+template <typename T>
+struct RosenbrockProblem {
+    const Problem<T>* P;
+    const OracleConfig* cfg;
+    int n;
+    const T* state;
+    const T* params;
+    const T* drivers;
+    T t, h, gamma;
+    T katol[MAXN], krtol[MAXN];
+    T inv_n;
+    void op(const T* v, T* out) const {
+        T jv[MAXN];
+        P->jvp(state, params, drivers, v, jv, t);
+        const T gh = gamma * h;
+        for (int i = 0; i < n; ++i) out[i] = v[i] - gh * jv[i];
+    }
+    void precond(const T* v, T* out) const {
+        const T h_eff = h * gamma;
+        for (int i = 0; i < n; ++i) out[i] = v[i];
+        for (int k = 0; k < cfg->preconditioner_order; ++k) {
+            T jv[MAXN];
+            P->jvp(state, params, drivers, out, jv, t);
+            for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * jv[i];
+        }
+    }
+    T krylov_norm2(const T* v) const {
+        T acc = T(0);
+        for (int i = 0; i < n; ++i) {
+            T tol = katol[i] + krtol[i] * std::fabs(state[i]);
+            tol = std::max(tol, T(1e-16));
+            T r = std::fabs(v[i]) / tol;
+            acc += r * r;
+        }
+        return acc * inv_n;
+    }
+};
+
+template <typename T>
+struct RosenbrockStepper {
+    const Problem<T>* P;
+    const OracleConfig* cfg;
+    int s, n, nd;
+    T A[MAXS][MAXS], C[MAXS][MAXS], b[MAXS], c[MAXS], d[MAXS], gamma_i[MAXS];
+    T gamma;
+    bool has_error, accumulates_output, accumulates_error;
+    int b_row, b_hat_row;
+    T stage_increment[MAXN];   // persists: last stage's K seeds the next step
+    T katol[MAXN], krtol[MAXN];
+
+    void init(const Problem<T>* p) {
+        P = p;
+        cfg = p->cfg;
+        s = cfg->stages;
+        n = p->n;
+        nd = cfg->n_drivers;
+        for (int i = 0; i < s; ++i) {
+            for (int j = 0; j < s; ++j) {
+                A[i][j] = T(cfg->a[i * s + j]);
+                C[i][j] = T(cfg->rb_C[i * s + j]);
+            }
+            b[i] = T(cfg->b[i]);
+            c[i] = T(cfg->c[i]);
+            gamma_i[i] = T(cfg->rb_gamma_stages[i]);
+        }
+        gamma = T(cfg->rb_gamma);
+        has_error = cfg->has_error != 0;
+        for (int i = 0; i < s; ++i) d[i] = has_error ? T(cfg->b[i] - cfg->b_hat[i]) : T(0);
+        b_row = ERKStepper<T>::find_row(cfg, cfg->b);
+        b_hat_row = has_error ? ERKStepper<T>::find_row(cfg, cfg->b_hat) : -1;
+        accumulates_output = b_row < 0;
+        accumulates_error = b_hat_row < 0;
+        for (int i = 0; i < MAXN; ++i) stage_increment[i] = T(0);
+        for (int i = 0; i < n; ++i) {
+            katol[i] = T(cfg->krylov_atol_v[i]);
+            krtol[i] = T(cfg->krylov_rtol_v[i]);
+        }
+    }
+
+    void time_derivative(const T* state, const T* params, const T* drivers, T* ddt, T* out,
+                         T current_time, T dt) const {
+        if (nd > 0) P->drivers_dt(current_time, ddt);
+        P->dfdt(state, params, drivers, ddt, out, current_time);
+        for (int i = 0; i < n; ++i) out[i] *= dt;
+    }
+
+    int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,
+                 T* observables, T* prop_obs, T* error, T dt, T time, bool first_step,
+                 bool accepted_flag, int32_t* counters) {
+        (void)first_step;
+        (void)accepted_flag;
+        const T current_time = time;
+        const T end_time = current_time + dt;
+        const T inv_dt = T(1) / dt;
+        T store[MAXS * MAXN], dfdt[MAXN], rhs[MAXN], obs_tmp[MAXN];
+        T ddt[16];
+        for (int q = 0; q < 16; ++q) ddt[q] = T(0);
+        time_derivative(state, params, drivers, ddt, dfdt, current_time, dt);
+        for (int i = 0; i < n; ++i) {
+            proposed[i] = state[i];
+            if (has_error) error[i] = T(0);
+        }
+        int32_t status = ST_SUCCESS;
+        RosenbrockProblem<T> rp;
+        rp.P = P; rp.cfg = cfg; rp.n = n; rp.state = state; rp.params = params;
+        rp.drivers = drivers; rp.h = dt; rp.gamma = gamma;
+        for (int i = 0; i < n; ++i) { rp.katol[i] = katol[i]; rp.krtol[i] = krtol[i]; }
+        rp.inv_n = T(1.0 / n);
+        int krylov_total = 0;
+
+        // stage 0 at the step start
+        T stage_time = current_time + dt * c[0];
+        P->rhs(state, params, drivers, observables, rhs, current_time);
+        for (int i = 0; i < n; ++i) {
+            T v = (rhs[i] + gamma_i[0] * dfdt[i]) * dt;
+            rhs[i] = v * gamma;
+        }
+        int it = 0;
+        rp.t = stage_time;
+        status |= linear_solve<T>(cfg, rp, n, rhs, stage_increment, false, it);
+        krylov_total = it;
+        for (int i = 0; i < n; ++i) store[i] = stage_increment[i];
+        for (int i = 0; i < n; ++i) {
+            if (accumulates_output) proposed[i] += stage_increment[i] * b[0];
+            if (has_error && accumulates_error) error[i] += stage_increment[i] * d[0];
+        }
+
+        for (int si = 1; si < s; ++si) {
+            stage_time = current_time + dt * c[si];
+            T Y[MAXN];
+            for (int i = 0; i < n; ++i) {
+                T v = state[i];
+                for (int j = 0; j < si; ++j) v += A[si][j] * store[j * n + i];
+                Y[i] = v;
+            }
+            if (nd > 0) P->drivers(stage_time, prop_drivers);
+            P->observables(Y, params, prop_drivers, prop_obs, stage_time);
+            P->rhs(Y, params, prop_drivers, obs_tmp, rhs, stage_time);
+            if (b_row == si)
+                for (int i = 0; i < n; ++i) proposed[i] = Y[i];
+            if (b_hat_row == si)
+                for (int i = 0; i < n; ++i) error[i] = Y[i];
+            for (int i = 0; i < n; ++i) {
+                T corr = T(0);
+                for (int j = 0; j < si; ++j) corr += C[si][j] * store[j * n + i];
+                T v = rhs[i] + corr * inv_dt + gamma_i[si] * dfdt[i];
+                rhs[i] = v * dt * gamma;
+            }
+            for (int i = 0; i < n; ++i) stage_increment[i] = store[(si - 1) * n + i];
+            rp.t = stage_time;
+            status |= linear_solve<T>(cfg, rp, n, rhs, stage_increment, false, it);
+            krylov_total = it;
+            for (int i = 0; i < n; ++i) store[si * n + i] = stage_increment[i];
+            for (int i = 0; i < n; ++i) {
+                if (accumulates_output) proposed[i] += b[si] * stage_increment[i];
+                if (has_error && accumulates_error) error[i] += d[si] * stage_increment[i];
+            }
+        }
+        if (!accumulates_error)
+            for (int i = 0; i < n; ++i) error[i] = proposed[i] - error[i];
+        if (nd > 0) P->drivers(end_time, prop_drivers);
+        P->observables(proposed, params, prop_drivers, prop_obs, end_time);
+        counters[0] = 0;
+        counters[1] = krylov_total;
+        return status;
+    }
+};
 
 }  // namespace oracle

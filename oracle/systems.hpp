@@ -32,6 +32,8 @@ typedef void (*obs_cb_t)(const double* y, const double* p, const double* d,
                          double* obs, double t);
 typedef void (*jvp_cb_t)(const double* y, const double* p, const double* d,
                          const double* v, double* jv, double t);
+typedef void (*dfdt_cb_t)(const double* y, const double* p, const double* d,
+                          const double* ddt, double* out, double t);
 
 struct SystemDesc {
     int id;
@@ -41,6 +43,7 @@ struct SystemDesc {
     rhs_cb_t rhs_cb;
     obs_cb_t obs_cb;
     jvp_cb_t jvp_cb;
+    dfdt_cb_t dfdt_cb;
 };
 
 // Hodgkin-Huxley constants (squid axon, resting potential -65 mV convention).
@@ -232,6 +235,32 @@ void sys_jvp(const SystemDesc& s, const T* y, const T* p, const T* d,
     }
     default:
         for (int i = 0; i < s.n; ++i) jv[i] = T(0);
+    }
+}
+
+// Total time derivative of f along the drivers: df/dt + sum_k df/dd_k * ddt_k
+// (what a Rosenbrock-W step feeds its gamma_i terms). Only Hodgkin-Huxley
+// reads a driver in its derivatives; the other systems are autonomous.
+template <typename T>
+void sys_dfdt(const SystemDesc& s, const T* y, const T* p, const T* d,
+              const T* ddt, T* out, T t) {
+    for (int i = 0; i < s.n; ++i) out[i] = T(0);
+    switch (s.id) {
+    case SYS_HODGKIN_HUXLEY:
+        out[0] = ddt[0] / T(HH_CM);
+        break;
+    case SYS_CALLBACK: {
+        if (!s.dfdt_cb) break;
+        double yd[64], pd[64], dd[16], dtd[16], od[64];
+        for (int i = 0; i < s.n; ++i) yd[i] = double(y[i]);
+        for (int i = 0; i < s.n_params; ++i) pd[i] = double(p[i]);
+        for (int i = 0; i < s.n_drivers; ++i) { dd[i] = double(d[i]); dtd[i] = double(ddt[i]); }
+        s.dfdt_cb(yd, pd, dd, dtd, od, double(t));
+        for (int i = 0; i < s.n; ++i) out[i] = T(od[i]);
+        break;
+    }
+    default:
+        break;
     }
 }
 

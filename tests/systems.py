@@ -1,0 +1,106 @@
+"""Test systems shared by the CPU and GPU tests (SymbolicODE definitions)."""
+
+import numpy as np
+
+import cubie_amd as ca
+
+LORENZ = ["dx = sigma*(y - x)", "dy = x*(rho - z) - y", "dz = x*y - beta*z"]
+
+HODGKIN_HUXLEY = """
+alpha_m = 0.1*(V + 40)/(1 - exp(-(V + 40)/10))
+beta_m = 4*exp(-(V + 65)/18)
+alpha_h = 0.07*exp(-(V + 65)/20)
+beta_h = 1/(1 + exp(-(V + 35)/10))
+alpha_n = 0.01*(V + 55)/(1 - exp(-(V + 55)/10))
+beta_n = 0.125*exp(-(V + 65)/80)
+dV = (I_ext - gNa*m**3*h*(V - ENa) - gK*n**4*(V - EK) - gL*(V - EL))/Cm
+dm = alpha_m*(1 - m) - beta_m*m
+dh = alpha_h*(1 - h) - beta_h*h
+dn = alpha_n*(1 - n) - beta_n*n
+"""
+
+ROBERTSON = """
+r1 = k1*y1
+r2 = k2*y2**2
+r3 = k3*y2*y3
+dy1 = -r1 + r3
+dy2 = r1 - r2 - r3
+dy3 = r2
+"""
+
+VAN_DER_POL = ["dx = y", "dy = mu*(1 - x**2)*y - x"]
+
+LINEAR3 = [
+    "dx0 = -x0", "dx1 = -x1/2", "dx2 = -x2/3",
+    "o0 = dx0 * p0 + c0 + d0", "o1 = dx1 * p1 + c1 + d0", "o2 = dx2 * p2 + c2 + d0",
+]
+
+
+def lorenz(precision=np.float64):
+    return ca.create_ODE_system(LORENZ, states={"x": 1.0, "y": 0.0, "z": 25.0},
+                                parameters={"sigma": 10.0, "rho": 28.0, "beta": 8.0 / 3.0},
+                                precision=precision, name="lorenz")
+
+
+def hodgkin_huxley(precision=np.float32):
+    return ca.create_ODE_system(
+        HODGKIN_HUXLEY, states={"V": -65.0, "m": 0.0529, "h": 0.5961, "n": 0.3177},
+        parameters={"gNa": 120.0, "gK": 36.0, "gL": 0.3},
+        constants={"ENa": 50.0, "EK": -77.0, "EL": -54.387, "Cm": 1.0},
+        drivers=["I_ext"], precision=precision, name="hodgkin_huxley")
+
+
+def robertson(precision=np.float64):
+    return ca.create_ODE_system(ROBERTSON, states={"y1": 1.0, "y2": 0.0, "y3": 0.0},
+                                parameters={"k1": 0.04, "k2": 3e7, "k3": 1e4},
+                                precision=precision, name="robertson")
+
+
+def van_der_pol(precision=np.float64):
+    return ca.create_ODE_system(VAN_DER_POL, states={"x": 2.0, "y": 0.0}, parameters={"mu": 1.0},
+                                precision=precision, name="van_der_pol")
+
+
+def linear3(precision=np.float64):
+    return ca.create_ODE_system(LINEAR3, states={"x0": 1.0, "x1": 1.0, "x2": 1.0},
+                                parameters={"p0": 1.0, "p1": 2.0, "p2": 3.0},
+                                constants={"c0": 0.5, "c1": 1.0, "c2": 2.0},
+                                drivers=["d0"], observables=["o0", "o1", "o2"],
+                                precision=precision, name="linear3")
+
+
+def hh_driver(seed=11, samples=1000, period=0.5):
+    rng = np.random.default_rng(seed)
+    t = np.arange(samples) * period
+    current = 10.0 * np.sin(2.0 * np.pi * t / 50.0) + rng.normal(0.0, 2.0, samples)
+    return {"I_ext": current, "dt": period, "order": 1, "wrap": False}
+
+
+def hh_batch(n, seed=12):
+    rng = np.random.default_rng(seed)
+    params = np.array([120.0, 36.0, 0.3])[:, None] * rng.uniform(0.7, 1.3, (3, n))
+    inits = np.tile(np.array([[-65.0], [0.0529], [0.5961], [0.3177]]), (1, n))
+    return inits, params
+
+
+def lorenz_batch(n, seed=1):
+    rng = np.random.default_rng(seed)
+    base = np.array([10.0, 28.0, 8.0 / 3.0])
+    params = base[:, None] * rng.uniform(0.9, 1.1, size=(3, n))
+    inits = rng.uniform(-1.0, 1.0, size=(3, n)) + np.array([0.0, 0.0, 25.0])[:, None]
+    return inits, params
+
+
+def robertson_batch(n, seed=21):
+    rng = np.random.default_rng(seed)
+    base = np.array([0.04, 3e7, 1e4])
+    params = base[:, None] * np.exp(rng.uniform(np.log(0.8), np.log(1.25), (3, n)))
+    inits = np.tile(np.array([[1.0], [0.0], [0.0]]), (1, n))
+    return inits, params
+
+
+def normwise_rel(a, b):
+    """max |a - b| / max |b| over each (variable, run) trajectory."""
+    num = np.abs(a - b).max(axis=0)
+    den = np.maximum(np.abs(b).max(axis=0), 1e-300)
+    return float((num / den).max())
