@@ -156,6 +156,11 @@ class BatchSolverKernel:
                 setattr(im, key, int(a[key]))
         if a.get("linear_correction_type") is not None:
             im.linear_correction_type = str(a["linear_correction_type"])
+        if a.get("preconditioner_type") is not None:
+            im.preconditioner_type = a["preconditioner_type"]
+            im.preconditioner_kinds  # validates the names
+        if not 1 <= im.preconditioner_order <= 32:
+            raise ValueError("preconditioner_order must lie in [1, 32]")
         return im
 
     def update(self, updates: Dict[str, Any]) -> set:

@@ -1,0 +1,112 @@
+// Single-stage implicit steps with their own starting guesses.
+//
+// BackwardEulerStep (integrators/algorithms/backwards_euler.py:212-300 and
+// backwards_euler_predict_correct.py:120-216): u = h f(t + h, y_n + u),
+// y_{n+1} = y_n + u. Without the predictor, Newton starts from the previous
+// step's increment; with it, from the explicit-Euler increment h f(t, y_n).
+//
+// CrankNicolsonStep (crank_nicolson.py:230-330): the CN increment solves
+// u = h f(t + h, y_n + h/2 f(y_n) + u/2) from the guess h f(y_n), then the
+// backward-Euler increment v = h f(t + h, y_n + v) is solved from u, and the
+// error estimate is y_CN - (y_n + v). Both solves share the Newton
+// contraction estimate and each overwrites the step counters.
+#pragma once
+
+namespace cubie {
+
+template <typename Cfg, typename Sys, typename Ctx>
+__device__ __forceinline__ int single_stage_solve(Ctx& ctx, real* u, const real* base, const real* state,
+                                                  real t, real h, real a, real& theta, int* counters) {
+    DirkStageProblem<Cfg, Sys> P;
+    P.u = u;
+    P.base = base;
+    P.step_start = state;
+    P.params = ctx.params.data();
+    P.drivers = ctx.proposed_drivers.data();
+    P.t = t;
+    P.h = h;
+    P.a = a;
+    return newton_solve<Cfg>(P, theta, counters);
+}
+
+template <typename Cfg, typename Sys>
+struct BackwardEulerStep {
+    static constexpr int n = Cfg::n;
+    Vec<real, n> increment;   // previous step's u (plain variant)
+    real theta;
+
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int i = 0; i < n; ++i) increment[i] = real(0);
+        theta = real(0);
+    }
+
+    template <typename Ctx>
+    __device__ __forceinline__ int step(Ctx& ctx, const Vec<real, n>& state, Vec<real, n>& proposed,
+                                        Vec<real, Cfg::n_error>& /*error*/, real dt, real time,
+                                        double /*t64*/, bool /*first_step*/, bool /*prev_accepted*/,
+                                        int* counters) {
+        real u[n];
+        if constexpr (Cfg::be_predictor) {
+            real f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1];
+            Sys::dxdt(state.data(), ctx.params.data(), ctx.drivers.data(), obs, f, time);
+#pragma unroll
+            for (int i = 0; i < n; ++i) u[i] = dt * f[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < n; ++i) u[i] = increment[i];
+        }
+        const real next_time = time + dt;
+        if constexpr (Cfg::n_drivers > 0) ctx.drv.eval(next_time, ctx.proposed_drivers.data());
+        const int status = single_stage_solve<Cfg, Sys>(ctx, u, state.data(), state.data(), next_time, dt,
+                                                        real(1), theta, counters);
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            if constexpr (!Cfg::be_predictor) increment[i] = u[i];
+            proposed[i] = u[i] + state[i];
+        }
+        if constexpr (Cfg::n_obs > 0)
+            Sys::observables(proposed.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                             ctx.proposed_observables.data(), next_time);
+        return status;
+    }
+};
+
+template <typename Cfg, typename Sys>
+struct CrankNicolsonStep {
+    static constexpr int n = Cfg::n;
+    real theta;
+
+    __device__ __forceinline__ void init() { theta = real(0); }
+
+    template <typename Ctx>
+    __device__ __forceinline__ int step(Ctx& ctx, const Vec<real, n>& state, Vec<real, n>& proposed,
+                                        Vec<real, Cfg::n_error>& error, real dt, real time,
+                                        double /*t64*/, bool /*first_step*/, bool /*prev_accepted*/,
+                                        int* counters) {
+        real f[n], base[n], u[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1];
+        Sys::dxdt(state.data(), ctx.params.data(), ctx.drivers.data(), obs, f, time);
+        const real half_dt = dt * real(0.5);
+        const real end_time = time + dt;
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            base[i] = state[i] + half_dt * f[i];
+            u[i] = dt * f[i];
+        }
+        if constexpr (Cfg::n_drivers > 0) ctx.drv.eval(end_time, ctx.proposed_drivers.data());
+        int status = single_stage_solve<Cfg, Sys>(ctx, u, base, state.data(), end_time, dt, real(0.5),
+                                                  theta, counters);
+#pragma unroll
+        for (int i = 0; i < n; ++i) proposed[i] = base[i] + real(0.5) * u[i];
+        status |= single_stage_solve<Cfg, Sys>(ctx, u, state.data(), state.data(), end_time, dt, real(1),
+                                               theta, counters);
+#pragma unroll
+        for (int i = 0; i < n; ++i) error[i] = proposed[i] - (state[i] + u[i]);
+        if constexpr (Cfg::n_obs > 0)
+            Sys::observables(proposed.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                             ctx.proposed_observables.data(), end_time);
+        return status;
+    }
+};
+
+}  // namespace cubie

@@ -1,2 +1,273 @@
-// Placeholder: the firk component is added with the implicit families.
+// Fully implicit Runge-Kutta step (Gauss-Legendre, Radau IIA).
+//
+// Follows the reference's FIRKStep.step (integrators/algorithms/
+// generic_firk.py:527-704): all s stages form one Newton problem of width
+// s*n whose unknown holds the stage increments z_i, with stage states
+// Y_i = y_n + sum_j a_ij z_j at times t + h c_i and one driver vector per
+// stage. The coupled residual, operator and Neumann / Jacobi
+// preconditioners are the n-stage forms of codegen/nonlinear_residuals.py:
+// 193-362, linear_operators.py:599-755 and preconditioners.py:344-399,
+// 927-1036; the Krylov norm tiles the step-start state over the stages
+// (norms.py:275-324) and the Newton norm measures each stage against its
+// own stage state (norms.py:377-432). The solution and error are the
+// compensated (Kahan) sums y_n + sum b_i z_i and sum d_i z_i, or copies of a
+// stage state when b / b_hat equals a row of A. The increments persist
+// between steps as the next Newton starting guess, read ahead by the dense
+// predictor when enabled. Tolerance vectors are indexed by state (entry e of
+// the coupled vector uses state e mod n).
 #pragma once
+
+namespace cubie {
+
+template <typename Cfg, typename Sys>
+struct FirkProblem {
+    static constexpr int n = Cfg::n;
+    static constexpr int s = Cfg::stages;
+    static constexpr int W = s * n;
+    static constexpr int nd = Cfg::n_drivers;
+    real* u;                 // coupled increments, stage-major
+    const real* base;        // y_n
+    const real* params;
+    const real* drivers;     // s stacked driver vectors
+    real t, h;
+
+    __device__ __forceinline__ real stage_time(int i) const { return t + h * Cfg::c[i]; }
+    __device__ __forceinline__ const real* stage_drivers(int i) const { return drivers + i * nd; }
+
+    // out = (with_base ? y_n : 0) + sum_j a_ij vec_j, skipping zero couplings.
+    template <bool with_base>
+    __device__ __forceinline__ void combine(int i, const real* vec, real* out) const {
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            real acc = with_base ? base[k] : real(0);
+#pragma unroll
+            for (int j = 0; j < s; ++j)
+                if (Cfg::A[i][j] != 0.0) acc += Cfg::A[i][j] * vec[j * n + k];
+            out[k] = acc;
+        }
+    }
+
+    __device__ __forceinline__ void op(const real* v, real* out) const {
+#pragma unroll
+        for (int i = 0; i < s; ++i) {
+            real Y[n], dir[n], jv[n];
+            combine<true>(i, u, Y);
+            combine<false>(i, v, dir);
+            Sys::jvp(Y, params, stage_drivers(i), dir, jv, stage_time(i));
+#pragma unroll
+            for (int k = 0; k < n; ++k) out[i * n + k] = v[i * n + k] - h * jv[k];
+        }
+    }
+    __device__ __forceinline__ void neumann(const real* v, real* out) const {
+#pragma unroll
+        for (int e = 0; e < W; ++e) out[e] = v[e];
+        for (int it = 0; it < Cfg::preconditioner_order; ++it) {
+            real jvp[W];
+#pragma unroll
+            for (int i = 0; i < s; ++i) {
+                real Y[n], dir[n];
+                combine<true>(i, u, Y);
+                combine<false>(i, out, dir);
+                Sys::jvp(Y, params, stage_drivers(i), dir, jvp + i * n, stage_time(i));
+            }
+#pragma unroll
+            for (int e = 0; e < W; ++e) out[e] = v[e] + h * jvp[e];
+        }
+    }
+    __device__ __forceinline__ void jacobi(const real* v, real* out) const {
+#pragma unroll
+        for (int i = 0; i < s; ++i) {
+            real Y[n], jd[n];
+            combine<true>(i, u, Y);
+            Sys::jac_diag(Y, params, stage_drivers(i), jd, stage_time(i));
+            const real scale = h * Cfg::A[i][i];
+#pragma unroll
+            for (int k = 0; k < n; ++k) out[i * n + k] = jacobi_divide(v[i * n + k], scale, jd[k]);
+        }
+    }
+    __device__ __forceinline__ void residual(real* out) const {
+#pragma unroll
+        for (int i = 0; i < s; ++i) {
+            real Y[n], f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1];
+            combine<true>(i, u, Y);
+            Sys::dxdt(Y, params, stage_drivers(i), obs, f, stage_time(i));
+#pragma unroll
+            for (int k = 0; k < n; ++k) out[i * n + k] = u[i * n + k] - h * f[k];
+        }
+    }
+    __device__ __forceinline__ real krylov_norm2(const real* vals) const {
+        real nrm2 = real(0);
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            const int k = e % n;
+            real tol = Cfg::krylov_atol[k] + Cfg::krylov_rtol[k] * dabs(base[k]);
+            tol = pmax(tol, real(1e-16));
+            const real r = dabs(vals[e]) / tol;
+            nrm2 += r * r;
+        }
+        return nrm2 * real(1.0 / double(W));
+    }
+    __device__ __forceinline__ real correction_norm2(const real* vals) const {
+        real nrm2 = real(0);
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            const int i = e / n, k = e % n;
+            real sv = base[k];
+#pragma unroll
+            for (int j = 0; j < s; ++j) sv += Cfg::A[i][j] * u[j * n + k];
+            const real ref = pmax(dabs(sv), dabs(base[k]));
+            real tol = Cfg::newton_atol[k] + Cfg::newton_rtol[k] * ref;
+            tol = pmax(tol, real(1e-16));
+            const real r = vals[e] / tol;
+            nrm2 += r * r;
+        }
+        return nrm2 * real(1.0 / double(W));
+    }
+};
+
+template <typename Cfg, typename Sys>
+struct FIRKStep {
+    static constexpr int n = Cfg::n;
+    static constexpr int s = Cfg::stages;
+    static constexpr int W = s * n;
+    static constexpr int nd = Cfg::n_drivers;
+    Vec<real, W> stage_increment;   // persists across steps
+    real previous_step_size;
+    real theta;
+
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int e = 0; e < W; ++e) stage_increment[e] = real(0);
+        previous_step_size = real(0);
+        theta = real(0);
+    }
+
+    // Dense read-ahead of the previous step's increments, in place
+    // (stage_predictors.py:492-548).
+    __device__ __forceinline__ void predict(real step_ratio, bool apply) {
+        if constexpr (Cfg::dense_prediction) {
+            constexpr int P = Cfg::pred_powers;
+            constexpr int R = Cfg::pred_rows;
+            constexpr int F = Cfg::pred_first;
+            real transform[R * s];
+#pragma unroll
+            for (int e = 0; e < R * s; ++e) {
+                real acc = real(0);
+#pragma unroll
+                for (int p = P - 1; p >= 0; --p) acc = acc * step_ratio + Cfg::pred_coeffs[p][e];
+                transform[e] = acc * step_ratio;
+            }
+#pragma unroll
+            for (int k = 0; k < n; ++k) {
+                real prev[s];
+#pragma unroll
+                for (int j = 0; j < s; ++j) prev[j] = stage_increment[j * n + k];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    real acc = real(0);
+#pragma unroll
+                    for (int j = 0; j < s; ++j) acc += transform[r * s + j] * prev[j];
+                    const int target = (F + r) * n + k;
+                    stage_increment[target] = apply ? acc : stage_increment[target];
+                }
+            }
+        }
+    }
+
+    template <typename Ctx>
+    __device__ __forceinline__ int step(Ctx& ctx, const Vec<real, n>& state, Vec<real, n>& proposed,
+                                        Vec<real, Cfg::n_error>& error, real dt, real time,
+                                        double /*t64*/, bool first_step, bool prev_accepted,
+                                        int* counters) {
+        const real current_time = time;
+        const real end_time = current_time + dt;
+        int status = ST_SUCCESS;
+        if constexpr (Cfg::dense_prediction) {
+            const real previous_dt = previous_step_size;
+            previous_step_size = dt;
+            const real safe_prev = previous_dt > real(0) ? previous_dt : dt;
+            const real step_ratio = dt / safe_prev;
+            const bool apply = (!first_step) && prev_accepted && (step_ratio <= Cfg::pred_max_ratio);
+            predict(step_ratio, apply);
+        }
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            if constexpr (Cfg::accumulates_output) proposed[k] = state[k];
+            if constexpr (Cfg::has_error && Cfg::accumulates_error) error[k] = real(0);
+        }
+        Vec<real, (nd > 0 ? s * nd : 1)> stack;
+        if constexpr (nd > 0) {
+#pragma unroll
+            for (int i = 0; i < s; ++i) ctx.drv.eval(current_time + dt * Cfg::c[i], stack.data() + i * nd);
+        }
+
+        FirkProblem<Cfg, Sys> P;
+        P.u = stage_increment.data();
+        P.base = state.data();
+        P.params = ctx.params.data();
+        P.drivers = stack.data();
+        P.t = current_time;
+        P.h = dt;
+        status |= newton_solve<Cfg>(P, theta, counters);
+
+#pragma unroll
+        for (int i = 0; i < s; ++i) {
+            if constexpr (nd > 0) {
+#pragma unroll
+                for (int q = 0; q < nd; ++q) ctx.proposed_drivers[q] = stack[i * nd + q];
+            }
+            const bool out_row = !Cfg::accumulates_output && Cfg::b_row == i;
+            const bool err_row = Cfg::has_error && !Cfg::accumulates_error && Cfg::b_hat_row == i;
+            if (out_row || err_row) {
+                real Y[n];
+                P.template combine<true>(i, stage_increment.data(), Y);
+#pragma unroll
+                for (int k = 0; k < n; ++k) {
+                    if (out_row) proposed[k] = Y[k];
+                    if constexpr (Cfg::has_error && !Cfg::accumulates_error) {
+                        if (err_row) error[k] = Y[k];
+                    }
+                }
+            }
+        }
+        if constexpr (Cfg::accumulates_output) {
+#pragma unroll
+            for (int k = 0; k < n; ++k) {
+                real acc = real(0), comp = real(0);
+#pragma unroll
+                for (int i = 0; i < s; ++i) {
+                    const real term = Cfg::b[i] * stage_increment[i * n + k] - comp;
+                    const real next = acc + term;
+                    comp = (next - acc) - term;
+                    acc = next;
+                }
+                proposed[k] = state[k] + acc;
+            }
+        }
+        if constexpr (Cfg::has_error && Cfg::accumulates_error) {
+#pragma unroll
+            for (int k = 0; k < n; ++k) {
+                real acc = real(0), comp = real(0);
+#pragma unroll
+                for (int i = 0; i < s; ++i) {
+                    const real term = Cfg::d[i] * stage_increment[i * n + k] - comp;
+                    const real next = acc + term;
+                    comp = (next - acc) - term;
+                    acc = next;
+                }
+                error[k] = acc;
+            }
+        }
+        if constexpr (Cfg::c[s - 1] != 1.0) ctx.drv.eval(end_time, ctx.proposed_drivers.data());
+        if constexpr (Cfg::n_obs > 0)
+            Sys::observables(proposed.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                             ctx.proposed_observables.data(), end_time);
+        if constexpr (Cfg::has_error && !Cfg::accumulates_error) {
+#pragma unroll
+            for (int k = 0; k < n; ++k) error[k] = proposed[k] - error[k];
+        }
+        return status;
+    }
+};
+
+}  // namespace cubie

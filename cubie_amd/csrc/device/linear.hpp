@@ -9,6 +9,7 @@
 //              (codegen/nonlinear_residuals.py:59-87)
 //   Neumann    out = v; repeat order: out = v + (gamma/beta)*h*a*J(out);
 //              out /= beta (codegen/preconditioners.py:85-142, 278-307)
+//   Jacobi     out = v / (beta - gamma*h*a*J_ii) (preconditioners.py:497-787)
 //   MR / SD    integrators/matrix_free_solvers/linear_solver.py:157-563
 //   BiCGSTAB   matrix_free_solvers/bicgstab_solver.py:226-672, with the same
 //              value clamps (1e16 / 1e150) and breakdown guards (1e-30 / 1e-200)
@@ -31,6 +32,37 @@
 namespace cubie {
 
 enum { LIN_MR = 0, LIN_SD = 1, LIN_BICGSTAB = 2 };
+enum { PRE_NEUMANN = 0, PRE_JACOBI = 1 };
+
+// Preconditioner application: the configured kinds in order (a single kind,
+// or a chain v -> P0 -> P1 -> ... as codegen/preconditioners.py:1113-1232
+// composes them), each supplied by the problem type as neumann() / jacobi().
+template <typename Cfg, typename Prob>
+__device__ __forceinline__ void precondition(const Prob& P, const real* v, real* out) {
+    constexpr int K = Cfg::precond_count;
+    if constexpr (K == 1) {
+        if constexpr (Cfg::precond_kinds[0] == PRE_JACOBI) P.jacobi(v, out);
+        else P.neumann(v, out);
+    } else {
+        real buf[2][Prob::W];
+        const real* src = v;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            real* dst = (k == K - 1) ? out : buf[k & 1];
+            if (Cfg::precond_kinds[k] == PRE_JACOBI) P.jacobi(src, dst);
+            else P.neumann(src, dst);
+            src = dst;
+        }
+    }
+}
+
+// One Jacobi entry: v / d with d = 1 - scale * J_ii, |d| floored at 1e-16
+// (codegen/preconditioners.py:554-594).
+__device__ __forceinline__ real jacobi_divide(real v, real scale, real jii) {
+    const real d = real(1) - scale * jii;
+    const real safe = dabs(d) >= real(1e-16) ? d : real(1e-16);
+    return v / safe;
+}
 
 template <typename real>
 struct SolverConsts;
@@ -82,7 +114,7 @@ __device__ __forceinline__ int mr_solve(Prob& P, real* rhs, real* x, bool x_is_z
     for (int k = 0; k < Cfg::krylov_max_iters; ++k) {
         if (converged) break;
         ++it;
-        P.precond(rhs, z);
+        precondition<Cfg>(P, rhs, z);
         P.op(z, temp);
         real num = real(0), den = real(0);
 #pragma unroll
@@ -142,7 +174,7 @@ __device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool 
     for (int k = 0; k < Cfg::krylov_max_iters; ++k) {
         if (finished) break;
         ++it;
-        P.precond(p, tmp);
+        precondition<Cfg>(P, p, tmp);
 #pragma unroll
         for (int i = 0; i < W; ++i) tmp[i] = clampc(tmp[i]);
         P.op(tmp, v);
@@ -165,7 +197,7 @@ __device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool 
         acc = P.krylov_norm2(rhs);
         converged = converged || (acc <= tol2);
         finished = converged || broken;
-        P.precond(rhs, s_hat);
+        precondition<Cfg>(P, rhs, s_hat);
 #pragma unroll
         for (int i = 0; i < W; ++i) s_hat[i] = clampc(s_hat[i]);
         P.op(s_hat, tmp);
@@ -304,7 +336,15 @@ struct DirkStageProblem {
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = v[i] - ah * jv[i];
     }
-    __device__ __forceinline__ void precond(const real* v, real* out) const {
+    __device__ __forceinline__ void jacobi(const real* v, real* out) const {
+        real Y[n], jd[n];
+        stage_state(Y);
+        Sys::jac_diag(Y, params, drivers, jd, t);
+        const real scale = h * a;
+#pragma unroll
+        for (int i = 0; i < n; ++i) out[i] = jacobi_divide(v[i], scale, jd[i]);
+    }
+    __device__ __forceinline__ void neumann(const real* v, real* out) const {
         const real h_eff = h * a;
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = v[i];

@@ -5,8 +5,12 @@ __init__.py: ``_ALGORITHM_REGISTRY`` plus every tableau alias) and the same
 controller defaults per family (generic_erk.py:77-88 ERK_ADAPTIVE_DEFAULTS,
 generic_dirk.py:69-104, generic_firk.py:71-104, generic_rosenbrock_w.py:
 71-106, backwards_euler.py:55, crank_nicolson.py:41, explicit_euler.py:41).
-The single-method families (euler, backwards_euler, crank_nicolson) are
-expressed as one- and two-stage tableaus of the generic steppers.
+Explicit Euler is the one-stage tableau of the generic ERK stepper (the
+arithmetic is identical). Backward Euler, its predictor-corrector variant
+and Crank-Nicolson keep their own steppers, as in the reference: their
+Newton starting guesses and Crank-Nicolson's two-solve error estimate are
+not expressible as a DIRK tableau. Their tableaus below carry only the
+metadata (order, nodes, whether an error estimate exists).
 """
 
 from __future__ import annotations
@@ -38,16 +42,16 @@ _FAMILY_BY_NAME = {
     "dirk": ("dirk", tb.DEFAULT_DIRK_TABLEAU),
     "firk": ("firk", tb.DEFAULT_FIRK_TABLEAU),
     "rosenbrock": ("rosenbrock", tb.DEFAULT_ROSENBROCK_TABLEAU),
-    "backwards_euler": ("dirk", BACKWARD_EULER),
-    "backwards_euler_pc": ("dirk", BACKWARD_EULER),
-    "crank_nicolson": ("dirk", CRANK_NICOLSON),
+    "backwards_euler": ("backwards_euler", BACKWARD_EULER),
+    "backwards_euler_pc": ("backwards_euler_pc", BACKWARD_EULER),
+    "crank_nicolson": ("crank_nicolson", CRANK_NICOLSON),
 }
 
 
 @dataclass(frozen=True)
 class ResolvedAlgorithm:
     name: str
-    family: str          # erk | dirk | firk | rosenbrock
+    family: str          # erk | dirk | firk | rosenbrock | backwards_euler[_pc] | crank_nicolson
     tableau: tb.ButcherTableau
 
     @property
@@ -113,5 +117,6 @@ ALL_ALGORITHM_STEP_PARAMETERS = {
     "algorithm", "tableau", "newton_atol", "newton_rtol", "newton_max_iters",
     "krylov_atol", "krylov_rtol", "krylov_max_iters", "krylov_residual_reduction",
     "krylov_residual_floor", "linear_correction_type", "preconditioner_order",
+    "preconditioner_type",
     "newton_damping", "newton_max_backtracks", "attempt_dense_prediction",
 }

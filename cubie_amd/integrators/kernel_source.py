@@ -14,7 +14,7 @@ from __future__ import annotations
 import hashlib
 import os
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -33,12 +33,17 @@ INCLUDE = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include")
 
 DEVICE_HEADERS = [
     "detmath.hpp", "common.hpp", "drivers.hpp", "controllers.hpp", "outputs.hpp", "linear.hpp",
-    "erk.hpp", "dirk.hpp", "firk.hpp", "rosenbrock.hpp", "loop.hpp",
+    "erk.hpp", "dirk.hpp", "firk.hpp", "rosenbrock.hpp", "euler_implicit.hpp", "loop.hpp",
 ]
 
 FAMILY_STEPPER = {
     "erk": "ERKStep", "dirk": "DIRKStep", "firk": "FIRKStep", "rosenbrock": "RosenbrockStep",
+    "backwards_euler": "BackwardEulerStep", "backwards_euler_pc": "BackwardEulerStep",
+    "crank_nicolson": "CrankNicolsonStep",
 }
+
+
+PRECONDITIONER_KINDS = {"neumann": 0, "jacobi": 1}
 
 
 @dataclass
@@ -57,7 +62,21 @@ class ImplicitSettings:
     krylov_residual_floor: Optional[float] = None
     linear_correction_type: str = "minimal_residual"
     preconditioner_order: int = 2
+    preconditioner_type: object = "neumann"
     attempt_dense_prediction: bool = True
+
+    @property
+    def preconditioner_kinds(self) -> Tuple[str, ...]:
+        """Preconditioners applied in order (a name or a sequence of names)."""
+        kinds = ((self.preconditioner_type,) if isinstance(self.preconditioner_type, str)
+                 else tuple(self.preconditioner_type))
+        for k in kinds:
+            if k not in PRECONDITIONER_KINDS:
+                raise ValueError(f"Unknown preconditioner type '{k}'; "
+                                 f"expected one of {sorted(PRECONDITIONER_KINDS)}")
+        if not kinds:
+            raise ValueError("preconditioner_type must name at least one preconditioner")
+        return kinds
 
     def tol(self, name: str, n: int) -> np.ndarray:
         arr = np.atleast_1d(np.asarray(getattr(self, name), dtype=np.float64))
@@ -209,6 +228,7 @@ class KernelSpec:
             a(f"  static constexpr real rbC[{st}][{st}] = {{{crow}}};")
             a(f"  static constexpr real gamma = {self._lit(tab.gamma)};")
             a(f"  static constexpr real gamma_stages[{st}] = {c_array(tab.gamma_stages, p)};")
+        a(f"  static constexpr bool be_predictor = {str(alg.family == 'backwards_euler_pc').lower()};")
         # implicit solver settings
         im = self.implicit
         lin = {"minimal_residual": 0, "steepest_descent": 1, "bicgstab": 2}[im.linear_correction_type]
@@ -216,6 +236,9 @@ class KernelSpec:
         a(f"  static constexpr int krylov_max_iters = {int(im.krylov_max_iters)};")
         a(f"  static constexpr int linear_solver = {lin};")
         a(f"  static constexpr int preconditioner_order = {int(im.preconditioner_order)};")
+        kinds = [PRECONDITIONER_KINDS[k] for k in im.preconditioner_kinds]
+        a(f"  static constexpr int precond_count = {len(kinds)};")
+        a(f"  static constexpr int precond_kinds[{len(kinds)}] = {c_int_array(kinds)};")
         for key in ("newton_atol", "newton_rtol", "krylov_atol", "krylov_rtol"):
             a(f"  static constexpr real {key}[{max(n, 1)}] = {c_array(im.tol(key, n), p)};")
         # dense stage prediction (DIRK / FIRK)

@@ -10,6 +10,7 @@ emitted as HIP C++ for hiprtc:
 * ``jvp(y, p, d, v, jv, t)``       J(y) v from the symbolic Jacobian,
                                    contracted with v (no dense J in memory)
 * ``dfdt(y, p, d, dd, out, t)``    df/dt with driver derivatives dd
+* ``jac_diag(y, p, d, Jd, t)``     diagonal of J (Jacobi preconditioner)
 * ``jac(y, p, d, J, t)``           dense J (row-major), for the adjoint
 * ``jacp(y, p, d, Jp, t)``         dense df/dparams (row-major)
 
@@ -290,6 +291,14 @@ class SystemCodegen:
             parts.append(f"    [[maybe_unused]] const real {_cname(str(dds[i]))} = dd[{i}];")
         parts += self._emit_block([(f"out[{i}]", e) for i, e in enumerate(dfdt)], "dfdt")
         parts.append("    (void)y; (void)p; (void)d; (void)dd; (void)t;")
+        parts.append("  }")
+        # Jacobian diagonal for the Jacobi preconditioner
+        Jd = [(f"Jd[{i}]", sp.diff(dx[i], s.state_symbols[i])) for i in range(n)]
+        parts.append("  __device__ static __forceinline__ void jac_diag(const real* __restrict__ y, "
+                     "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ Jd, real t) {")
+        parts += self._bindings()
+        parts += self._emit_block(Jd, "jac_diag")
+        parts.append("    (void)y; (void)p; (void)d; (void)Jd; (void)t;")
         parts.append("  }")
         # dense Jacobians for the adjoint
         J = [(f"J[{i * n + j}]", sp.diff(dx[i], s.state_symbols[j])) for i in range(n) for j in range(n)]

@@ -367,6 +367,18 @@ class SymbolicODE:
         jvp = sp.lambdify(args + (v,), jv, modules=mods)
         return rhs, obs, jvp
 
+    def numpy_dfdt(self):
+        """NumPy callable for df/dt + sum_k df/dd_k * dd_k/dt (CPU checks)."""
+        args = (self.state_symbols, self.parameter_symbols, self.driver_symbols, TIME_SYMBOL)
+        const_sub = {sp.Symbol(k, real=True): v for k, v in self._constants.items()}
+        f = [e.xreplace(const_sub) for e in self.derivative_exprs()]
+        dd = [sp.Symbol(f"_dd{i}", real=True) for i in range(self.num_drivers)]
+        out = [sp.diff(fi, TIME_SYMBOL) + sum(sp.diff(fi, d) * ddi
+                                              for d, ddi in zip(self.driver_symbols, dd))
+               for fi in f]
+        mods = [{"Heaviside": lambda x: np.heaviside(x, 0.5)}, "numpy"]
+        return sp.lambdify(args + (dd,), out, modules=mods)
+
     def __repr__(self) -> str:
         return (f"SymbolicODE({self.name!r}, states={self.state_names}, "
                 f"parameters={self.parameter_names}, drivers={self.driver_names}, "

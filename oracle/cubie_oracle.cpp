@@ -48,7 +48,8 @@ enum : int32_t {
     ST_NEWTON_DIVERGENCE = 256,
 };
 
-enum Family { FAM_ERK = 0, FAM_DIRK = 1, FAM_FIRK = 2, FAM_ROSENBROCK = 3 };
+enum Family { FAM_ERK = 0, FAM_DIRK = 1, FAM_FIRK = 2, FAM_ROSENBROCK = 3,
+              FAM_BACKWARD_EULER = 4, FAM_CRANK_NICOLSON = 5 };
 enum Controller { CTRL_FIXED = 0, CTRL_I = 1, CTRL_PI = 2, CTRL_PID = 3, CTRL_GUSTAFSSON = 4 };
 
 // Summary metric ids (registration order of the reference's built-ins,
@@ -147,6 +148,11 @@ struct OracleConfig {
     double pred_max_ratio;
     // time derivative callback (SYS_CALLBACK, Rosenbrock-W); may be null
     dfdt_cb_t dfdt_cb;
+    // preconditioner chain (0 Neumann, 1 Jacobi), applied in order
+    int32_t precond_count;
+    const int32_t* precond_kinds;
+    // backward Euler: 0 warm start from the last increment, 1 explicit-Euler predictor
+    int32_t be_predictor;
 };
 
 }  // extern "C"
@@ -979,6 +985,8 @@ static int dispatch_family(const OracleConfig* cfg, int64_t n_runs, const double
     case FAM_DIRK: return solve_typed<T, DIRKStepper<T>>(cfg, n_runs, inits, params, out, b, e);
     case FAM_FIRK: return solve_typed<T, FIRKStepper<T>>(cfg, n_runs, inits, params, out, b, e);
     case FAM_ROSENBROCK: return solve_typed<T, RosenbrockStepper<T>>(cfg, n_runs, inits, params, out, b, e);
+    case FAM_BACKWARD_EULER: return solve_typed<T, BackwardEulerStepper<T>>(cfg, n_runs, inits, params, out, b, e);
+    case FAM_CRANK_NICOLSON: return solve_typed<T, CrankNicolsonStepper<T>>(cfg, n_runs, inits, params, out, b, e);
     default: return -2;
     }
 }

@@ -23,8 +23,10 @@ LIB = os.path.join(HERE, "_build", "liboracle.so")
 
 SYSTEM_IDS = {"lorenz": 0, "lorenz_julia": 1, "robertson": 2, "hodgkin_huxley": 3,
               "van_der_pol": 4, "linear3": 5, "callback": 6, "decay": 7}
-FAMILY_IDS = {"erk": 0, "dirk": 1, "firk": 2, "rosenbrock": 3}
+FAMILY_IDS = {"erk": 0, "dirk": 1, "firk": 2, "rosenbrock": 3, "backwards_euler": 4,
+              "backwards_euler_pc": 4, "crank_nicolson": 5}
 LINEAR_IDS = {"minimal_residual": 0, "steepest_descent": 1, "bicgstab": 2}
+PRECOND_IDS = {"neumann": 0, "jacobi": 1}
 
 RHS_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -35,6 +37,10 @@ OBS_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(
 JVP_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double), ctypes.c_double)
+
+DFDT_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                           ctypes.POINTER(ctypes.c_double), ctypes.c_double)
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -86,6 +92,9 @@ class OracleConfig(ctypes.Structure):
         ("krylov_atol_v", _D), ("krylov_rtol_v", _D), ("newton_atol_v", _D), ("newton_rtol_v", _D),
         ("dense_prediction", ctypes.c_int32), ("pred_powers", ctypes.c_int32),
         ("pred_first", ctypes.c_int32), ("pred_coeffs", _D), ("pred_max_ratio", ctypes.c_double),
+        ("dfdt_cb", DFDT_CB),
+        ("precond_count", ctypes.c_int32), ("precond_kinds", _I),
+        ("be_predictor", ctypes.c_int32),
     ]
 
 
@@ -209,6 +218,7 @@ class OracleRun:
         cfg.constants = _dp(arr(consts))
         cfg.n_constants = len(consts)
         cfg.family = FAMILY_IDS[alg.family]
+        cfg.be_predictor = int(alg.family == "backwards_euler_pc")
         s = tab.stage_count
         cfg.stages = s
         cfg.order = tab.order
@@ -289,6 +299,9 @@ class OracleRun:
         cfg.krylov_max_iters = im.krylov_max_iters
         cfg.linear_solver = LINEAR_IDS[im.linear_correction_type]
         cfg.preconditioner_order = im.preconditioner_order
+        kinds = [PRECOND_IDS[k] for k in im.preconditioner_kinds]
+        cfg.precond_count = len(kinds)
+        cfg.precond_kinds = _ip(arr(kinds, np.int32))
         cfg.newton_atol_s = float(np.min(im.newton_atol))
         cfg.newton_rtol_s = float(np.min(im.newton_rtol))
         cfg.krylov_atol_s = float(np.min(im.krylov_atol))
@@ -299,12 +312,14 @@ class OracleRun:
         cfg.krylov_floor = (im.krylov_residual_floor if im.krylov_residual_floor is not None
                             else float(np.sqrt(np.finfo(prec).eps)))
         if callbacks is not None:
-            cfg.rhs_cb, cfg.obs_cb, cfg.jvp_cb = callbacks
+            cfg.rhs_cb, cfg.obs_cb, cfg.jvp_cb = callbacks[:3]
+            cfg.dfdt_cb = callbacks[3] if len(callbacks) > 3 else DFDT_CB()
             keep.append(callbacks)
         else:
             cfg.rhs_cb = RHS_CB()
             cfg.obs_cb = OBS_CB()
             cfg.jvp_cb = JVP_CB()
+            cfg.dfdt_cb = DFDT_CB()
         cfg.deterministic_pow = int(getattr(k, "deterministic_pow", False))
         n = sysm.num_states
         cfg.krylov_atol_v = _dp(arr(im.tol("krylov_atol", n)))
@@ -399,4 +414,12 @@ def callbacks_for(system):
         for i in range(n):
             jv[i] = float(vals[i])
 
-    return RHS_CB(_r), OBS_CB(_o), JVP_CB(_j)
+    dfdt = system.numpy_dfdt()
+
+    def _t(y, p, d, dd, out, t):
+        vals = dfdt([y[i] for i in range(n)], [p[i] for i in range(p_)], [d[i] for i in range(nd)], t,
+                    [dd[i] for i in range(nd)])
+        for i in range(n):
+            out[i] = float(vals[i])
+
+    return RHS_CB(_r), OBS_CB(_o), JVP_CB(_j), DFDT_CB(_t)

@@ -16,6 +16,48 @@
 
 namespace oracle {
 
+static const int MAXW = 4 * MAXN;   // widest coupled vector the solvers hold
+
+// Preconditioner kinds, applied in sequence when several are configured
+// (codegen/preconditioners.py:1113-1232 chained composition: v -> P0 -> P1 ...).
+enum { PRE_NEUMANN = 0, PRE_JACOBI = 1 };
+
+template <typename T, typename Prob>
+static void precondition(const OracleConfig* cfg, const Prob& Pb, const T* v, T* out) {
+    const int K = cfg->precond_count > 0 ? cfg->precond_count : 1;
+    T buf[2][MAXW];
+    const T* src = v;
+    for (int k = 0; k < K; ++k) {
+        T* dst = (k == K - 1) ? out : buf[k & 1];
+        const int kind = cfg->precond_count > 0 ? cfg->precond_kinds[k] : PRE_NEUMANN;
+        if (kind == PRE_JACOBI) Pb.jacobi(src, dst);
+        else Pb.neumann(src, dst);
+        src = dst;
+    }
+}
+
+// Diagonal of J(Y) from unit-vector Jacobian-vector products.
+template <typename T>
+static void jacobian_diagonal(const Problem<T>* P, const T* Y, const T* params, const T* drivers,
+                              T t, T* diag) {
+    const int n = P->n;
+    for (int i = 0; i < n; ++i) {
+        T e[MAXN], jv[MAXN];
+        for (int k = 0; k < n; ++k) e[k] = T(k == i ? 1 : 0);
+        P->jvp(Y, params, drivers, e, jv, t);
+        diag[i] = jv[i];
+    }
+}
+
+// Jacobi step out = v / d with d = 1 - scale * J_ii, magnitude floored at 1e-16
+// (codegen/preconditioners.py:554-594 guarded division).
+template <typename T>
+static inline T jacobi_divide(T v, T scale, T jii) {
+    T d = T(1) - scale * jii;
+    T safe = std::fabs(d) >= T(1e-16) ? d : T(1e-16);
+    return v / safe;
+}
+
 template <typename T> struct SC;
 template <> struct SC<double> {
     static constexpr double clamp = 1e150, breakdown = 1e-200;
@@ -59,7 +101,15 @@ struct StageProblem {
         T ah = a * h;
         for (int i = 0; i < n; ++i) out[i] = v[i] - ah * jv[i];
     }
-    void precond(const T* v, T* out) const {
+    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
+    void jacobi(const T* v, T* out) const {
+        T Y[MAXN], dg[MAXN];
+        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
+        jacobian_diagonal<T>(P, Y, params, drivers, t, dg);
+        const T scale = h * a;
+        for (int i = 0; i < n; ++i) out[i] = jacobi_divide<T>(v[i], scale, dg[i]);
+    }
+    void neumann(const T* v, T* out) const {
         T h_eff = h * a;
         for (int i = 0; i < n; ++i) out[i] = v[i];
         for (int k = 0; k < cfg->preconditioner_order; ++k) {
@@ -104,7 +154,7 @@ static int32_t mr_solve(const OracleConfig* cfg, Prob& Pb, int W, T* rhs, T* x, 
     T rn2 = Pb.krylov_norm2(rhs);
     T tol = flo + red * T(std::sqrt(rn2));
     T tol2 = tol * tol;
-    T temp[4 * MAXN], z[4 * MAXN];
+    T temp[MAXW], z[MAXW];
     if (!x_zero) {
         Pb.op(x, temp);
         for (int i = 0; i < W; ++i) rhs[i] = rhs[i] - temp[i];
@@ -139,7 +189,7 @@ static int32_t bicgstab_solve(const OracleConfig* cfg, Prob& Pb, int W, T* rhs, 
     T rn2 = Pb.krylov_norm2(rhs);
     T tol = flo + red * T(std::sqrt(rn2));
     T tol2 = tol * tol;
-    T r0[4 * MAXN], p[4 * MAXN], v[4 * MAXN], tmp[4 * MAXN], sh[4 * MAXN];
+    T r0[MAXW], p[MAXW], v[MAXW], tmp[MAXW], sh[MAXW];
     if (!x_zero) Pb.op(x, tmp);
     else for (int i = 0; i < W; ++i) tmp[i] = T(0);
     T rho_prev = T(0);
@@ -223,7 +273,7 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
     bool conv = false, failed = false;
     int32_t last_lin = ST_SUCCESS;
     int iters = 0, ktot = 0, iteration = 0;
-    T res[4 * MAXN], delta[4 * MAXN];
+    T res[MAXW], delta[MAXW];
     for (int k = 0; k < cfg->newton_max_iters; ++k) {
         if (conv || failed) break;
         ++iteration;
@@ -478,8 +528,6 @@ struct UnsupportedStepper {
 // Stage i of the unknown u holds z_i = Y_i - y_n, with Y_i = y_n + sum_j a_ij z_j
 // (the tableau's a, not the a_ij argument, which the coupled solve passes as 0).
 // --------------------------------------------------------------------------
-static const int MAXW = 4 * MAXN;   // widest coupled vector the solvers hold
-
 template <typename T>
 struct FIRKProblem {
     const Problem<T>* P;
@@ -516,7 +564,17 @@ struct FIRKProblem {
             for (int k = 0; k < n; ++k) out[i * n + k] = v[i * n + k] - h * jv[k];
         }
     }
-    void precond(const T* v, T* out) const {
+    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
+    void jacobi(const T* v, T* out) const {
+        for (int i = 0; i < s; ++i) {
+            T Y[MAXN], dg[MAXN];
+            combine(i, u, Y, true);
+            jacobian_diagonal<T>(P, Y, params, stage_drivers(i), stage_time(i), dg);
+            const T scale = h * A[i][i];
+            for (int k = 0; k < n; ++k) out[i * n + k] = jacobi_divide<T>(v[i * n + k], scale, dg[k]);
+        }
+    }
+    void neumann(const T* v, T* out) const {
         const int W = s * n;
         for (int e = 0; e < W; ++e) out[e] = v[e];
         for (int k = 0; k < cfg->preconditioner_order; ++k) {
@@ -729,7 +787,14 @@ struct RosenbrockProblem {
         const T gh = gamma * h;
         for (int i = 0; i < n; ++i) out[i] = v[i] - gh * jv[i];
     }
-    void precond(const T* v, T* out) const {
+    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
+    void jacobi(const T* v, T* out) const {
+        T dg[MAXN];
+        jacobian_diagonal<T>(P, state, params, drivers, t, dg);
+        const T scale = gamma * h;
+        for (int i = 0; i < n; ++i) out[i] = jacobi_divide<T>(v[i], scale, dg[i]);
+    }
+    void neumann(const T* v, T* out) const {
         const T h_eff = h * gamma;
         for (int i = 0; i < n; ++i) out[i] = v[i];
         for (int k = 0; k < cfg->preconditioner_order; ++k) {
@@ -830,9 +895,9 @@ struct RosenbrockStepper {
             rhs[i] = v * gamma;
         }
         int it = 0;
-        rp.t = stage_time;
+        rp.t = current_time;   // J is taken at the step start
         status |= linear_solve<T>(cfg, rp, n, rhs, stage_increment, false, it);
-        krylov_total = it;
+        krylov_total += it;
         for (int i = 0; i < n; ++i) store[i] = stage_increment[i];
         for (int i = 0; i < n; ++i) {
             if (accumulates_output) proposed[i] += stage_increment[i] * b[0];
@@ -861,9 +926,8 @@ struct RosenbrockStepper {
                 rhs[i] = v * dt * gamma;
             }
             for (int i = 0; i < n; ++i) stage_increment[i] = store[(si - 1) * n + i];
-            rp.t = stage_time;
             status |= linear_solve<T>(cfg, rp, n, rhs, stage_increment, false, it);
-            krylov_total = it;
+            krylov_total += it;
             for (int i = 0; i < n; ++i) store[si * n + i] = stage_increment[i];
             for (int i = 0; i < n; ++i) {
                 if (accumulates_output) proposed[i] += b[si] * stage_increment[i];
@@ -874,8 +938,125 @@ struct RosenbrockStepper {
             for (int i = 0; i < n; ++i) error[i] = proposed[i] - error[i];
         if (nd > 0) P->drivers(end_time, prop_drivers);
         P->observables(proposed, params, prop_drivers, prop_obs, end_time);
-        counters[0] = 0;
-        counters[1] = krylov_total;
+        // The reference step never writes the loop's counters for this family,
+        // so Newton / Krylov counts stay at the zeros the loop started with.
+        (void)counters;
+        (void)krylov_total;
+        return status;
+    }
+};
+
+// --------------------------------------------------------------------------
+// Backward Euler (integrators/algorithms/backwards_euler.py:212-300) and its
+// predictor-corrector variant (backwards_euler_predict_correct.py:120-216):
+// u = h f(t + h, y_n + u), y_{n+1} = y_n + u. The plain variant starts Newton
+// from the previous step's increment, the predictor variant from h f(t, y_n).
+// --------------------------------------------------------------------------
+template <typename T>
+struct BackwardEulerStepper {
+    const Problem<T>* P;
+    const OracleConfig* cfg;
+    int n;
+    bool predictor;
+    T increment[MAXN];
+    T theta;
+    T katol[MAXN], krtol[MAXN], natol[MAXN], nrtol[MAXN];
+
+    void init(const Problem<T>* p) {
+        P = p;
+        cfg = p->cfg;
+        n = p->n;
+        predictor = cfg->be_predictor != 0;
+        for (int i = 0; i < MAXN; ++i) increment[i] = T(0);
+        theta = T(0);
+        fill_inner_tols<T>(cfg, n, katol, krtol, natol, nrtol);
+    }
+
+    int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,
+                 T* observables, T* prop_obs, T* error, T dt, T time, bool, bool,
+                 int32_t* counters) {
+        (void)error;
+        T u[MAXN];
+        if (predictor) {
+            T f[MAXN];
+            P->rhs(state, params, drivers, observables, f, time);
+            for (int i = 0; i < n; ++i) u[i] = dt * f[i];
+        } else {
+            for (int i = 0; i < n; ++i) u[i] = increment[i];
+        }
+        const T next_time = time + dt;
+        if (cfg->n_drivers > 0) P->drivers(next_time, prop_drivers);
+        StageProblem<T> sp;
+        sp.P = P; sp.cfg = cfg; sp.n = n; sp.u = u; sp.base = state; sp.step_start = state;
+        sp.params = params; sp.drivers = prop_drivers; sp.t = next_time; sp.h = dt; sp.a = T(1);
+        for (int i = 0; i < n; ++i) {
+            sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
+        }
+        sp.inv_n = T(1.0 / n);
+        int32_t status = newton_solve<T>(cfg, sp, n, theta, counters);
+        for (int i = 0; i < n; ++i) {
+            if (!predictor) increment[i] = u[i];
+            proposed[i] = u[i] + state[i];
+        }
+        P->observables(proposed, params, prop_drivers, prop_obs, next_time);
+        return status;
+    }
+};
+
+// --------------------------------------------------------------------------
+// Crank-Nicolson with a backward-Euler error estimate
+// (integrators/algorithms/crank_nicolson.py:230-330): the CN increment solves
+// u = h f(t + h, y_n + h/2 f(y_n) + u/2) from the guess h f(y_n); the BE
+// increment v = h f(t + h, y_n + v) starts from u; error = y_CN - (y_n + v).
+// --------------------------------------------------------------------------
+template <typename T>
+struct CrankNicolsonStepper {
+    const Problem<T>* P;
+    const OracleConfig* cfg;
+    int n;
+    T theta;
+    T katol[MAXN], krtol[MAXN], natol[MAXN], nrtol[MAXN];
+
+    void init(const Problem<T>* p) {
+        P = p;
+        cfg = p->cfg;
+        n = p->n;
+        theta = T(0);
+        fill_inner_tols<T>(cfg, n, katol, krtol, natol, nrtol);
+    }
+
+    int32_t solve(T* u, const T* base, const T* state, const T* params, const T* drivers, T t,
+                  T h, T a, int32_t* counters) {
+        StageProblem<T> sp;
+        sp.P = P; sp.cfg = cfg; sp.n = n; sp.u = u; sp.base = base; sp.step_start = state;
+        sp.params = params; sp.drivers = drivers; sp.t = t; sp.h = h; sp.a = a;
+        for (int i = 0; i < n; ++i) {
+            sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
+        }
+        sp.inv_n = T(1.0 / n);
+        return newton_solve<T>(cfg, sp, n, theta, counters);
+    }
+
+    int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,
+                 T* observables, T* prop_obs, T* error, T dt, T time, bool, bool,
+                 int32_t* counters) {
+        T f[MAXN], base[MAXN], u[MAXN], v[MAXN];
+        P->rhs(state, params, drivers, observables, f, time);
+        const T half_dt = dt * T(0.5);
+        const T end_time = time + dt;
+        for (int i = 0; i < n; ++i) {
+            base[i] = state[i] + half_dt * f[i];
+            u[i] = dt * f[i];
+        }
+        if (cfg->n_drivers > 0) P->drivers(end_time, prop_drivers);
+        int32_t status = solve(u, base, state, params, prop_drivers, end_time, dt, T(0.5), counters);
+        for (int i = 0; i < n; ++i) {
+            proposed[i] = base[i] + T(0.5) * u[i];
+            v[i] = u[i];
+        }
+        status |= solve(v, state, state, params, prop_drivers, end_time, dt, T(1), counters);
+        for (int i = 0; i < n; ++i) error[i] = proposed[i] - (state[i] + v[i]);
+        P->observables(proposed, params, prop_drivers, prop_obs, end_time);
         return status;
     }
 };

@@ -104,3 +104,11 @@ def normwise_rel(a, b):
     num = np.abs(a - b).max(axis=0)
     den = np.maximum(np.abs(b).max(axis=0), 1e-300)
     return float((num / den).max())
+
+
+def lorenz_julia(precision=np.float32):
+    """Lorenz with rho as the only parameter (the Julia-gate ensemble)."""
+    return ca.create_ODE_system(LORENZ, states={"x": 1.0, "y": 0.0, "z": 0.0},
+                                parameters={"rho": 21.0},
+                                constants={"sigma": 10.0, "beta": 8.0 / 3.0},
+                                precision=precision, name="lorenz_julia")

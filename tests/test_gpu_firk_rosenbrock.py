@@ -1,0 +1,125 @@
+"""GPU parity of the coupled FIRK and Rosenbrock-W steppers and of the
+Jacobi / chained preconditioners, against the oracle on the same inputs.
+
+Like the DIRK tests, the generated HIP right-hand sides and the oracle's
+hand-written ones round differently, so results agree to the accumulated
+rounding of the stage solves; the bars are stated per test.
+"""
+
+import numpy as np
+import pytest
+
+import cubie_amd as ca
+
+from systems import (hh_batch, hh_driver, hodgkin_huxley, normwise_rel, robertson,
+                     robertson_batch, van_der_pol)
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(solver, oracle_module, system_id, inits, params, duration):
+    res = solver.solve(inits, params, duration=duration, nan_error_trajectories=False)
+    run = oracle_module.OracleRun(solver.kernel, system_id, duration)
+    ref = run.solve(inits, params, threads=8)
+    return res, ref
+
+
+def check_counts(res, ref, rel=0.01):
+    """Accepted / rejected step totals within rel of the oracle's."""
+    for k in (2, 3):
+        g = int(res.iteration_counters[:, k].sum())
+        o = int(ref["iteration_counters"][:, k].sum())
+        assert abs(g - o) <= rel * o + 2, (k, g, o)
+
+
+@pytest.mark.parametrize("alg", ["radau", "ros3p", "rodas3p", "rosenbrock23"])
+def test_robertson_fp64(oracle_module, alg):
+    """Robertson kinetics, 64 systems over t in [0, 10], rtol 1e-6,
+    atol 1e-10, BiCGSTAB. Bar: statuses equal and zero, saved states within
+    1e-8 normwise, accepted and rejected step totals within 1 %."""
+    n = 64
+    inits, params = robertson_batch(n)
+    solver = ca.Solver(robertson(np.float64), algorithm=alg, atol=1e-10, rtol=1e-6,
+                       linear_correction_type="bicgstab", save_every=0.5,
+                       output_types=["state", "iteration_counters"])
+    res, ref = run_both(solver, oracle_module, "robertson", inits, params, 10.0)
+    assert np.all(res.status_codes == 0), res.status_messages
+    assert np.array_equal(res.status_codes, ref["status_codes"])
+    err = normwise_rel(res.state, ref["state"])
+    print(alg, "normwise rel err", err)
+    assert err <= 1e-8
+    check_counts(res, ref)
+
+
+@pytest.mark.parametrize("alg", ["radau", "ros3p"])
+def test_robertson_fp32(oracle_module, alg):
+    """Single precision at rtol 1e-4, atol 1e-8: states within 1e-3 normwise."""
+    n = 64
+    inits, params = robertson_batch(n, seed=4)
+    solver = ca.Solver(robertson(np.float32), algorithm=alg, atol=1e-8, rtol=1e-4,
+                       linear_correction_type="bicgstab", save_every=0.5,
+                       output_types=["state", "iteration_counters"])
+    res, ref = run_both(solver, oracle_module, "robertson", inits, params, 5.0)
+    assert np.array_equal(res.status_codes, ref["status_codes"])
+    err = normwise_rel(res.state, ref["state"])
+    print(alg, "fp32 normwise rel err", err)
+    assert err <= 1e-3
+
+
+@pytest.mark.parametrize("pre,lin", [("jacobi", "minimal_residual"),
+                                     (("jacobi", "neumann"), "bicgstab"),
+                                     (("neumann", "jacobi"), "bicgstab")])
+@pytest.mark.parametrize("alg", ["radau", "ros3p", "kvaerno5"])
+def test_preconditioners_robertson(oracle_module, alg, pre, lin):
+    """Jacobi and chained preconditioners. Minimal residual converges here
+    with the Jacobi scaling; the Neumann series stalls minimal residual on
+    this stiff Jacobian (also behind a Jacobi stage), so the chains run under
+    BiCGSTAB."""
+    n = 64
+    inits, params = robertson_batch(n, seed=8)
+    solver = ca.Solver(robertson(np.float64), algorithm=alg, atol=1e-10, rtol=1e-6,
+                       linear_correction_type=lin, preconditioner_type=pre,
+                       save_every=0.25, output_types=["state", "iteration_counters"])
+    res, ref = run_both(solver, oracle_module, "robertson", inits, params, 2.0)
+    assert np.array_equal(res.status_codes, ref["status_codes"])
+    err = normwise_rel(res.state, ref["state"])
+    print(alg, pre, "normwise rel err", err)
+    assert err <= 1e-8
+
+
+@pytest.mark.parametrize("alg", ["firk_gauss_legendre_2", "firk_gauss_legendre_4", "radau"])
+def test_firk_van_der_pol_fixed(oracle_module, alg):
+    """Fixed-step FIRK on a non-stiff van der Pol batch (mu in [0.5, 2])."""
+    n = 64
+    rng = np.random.default_rng(31)
+    inits = rng.uniform(-2.0, 2.0, (2, n))
+    params = rng.uniform(0.5, 2.0, (1, n))
+    solver = ca.Solver(van_der_pol(np.float64), algorithm=alg, dt=0.01,
+                       step_controller="fixed", linear_correction_type="bicgstab",
+                       save_every=0.5, output_types=["state", "iteration_counters"])
+    res, ref = run_both(solver, oracle_module, "van_der_pol", inits, params, 10.0)
+    assert np.all(res.status_codes == 0), res.status_messages
+    err = normwise_rel(res.state, ref["state"])
+    print(alg, "normwise rel err", err)
+    assert err <= 1e-9
+    for k in (0, 1):
+        g = int(res.iteration_counters[:, k].sum())
+        o = int(ref["iteration_counters"][:, k].sum())
+        assert abs(g - o) <= 0.02 * o + 2
+
+
+def test_hh_ros3p_driver(oracle_module):
+    """Rosenbrock on the driven HH neuron: exercises the driver time
+    derivative (dd/dt of the interpolated current) in the gamma_i terms."""
+    n = 64
+    inits, params = hh_batch(n, seed=17)
+    solver = ca.Solver(hodgkin_huxley(np.float64), algorithm="ros3p", atol=1e-8, rtol=1e-6,
+                       linear_correction_type="bicgstab", save_every=1.0,
+                       output_types=["state", "iteration_counters"])
+    solver._configure_drivers(hh_driver())
+    res, ref = run_both(solver, oracle_module, "hodgkin_huxley", inits, params, 50.0)
+    assert np.all(res.status_codes == 0), res.status_messages
+    err = normwise_rel(res.state, ref["state"])
+    print("hh ros3p normwise rel err", err)
+    assert err <= 1e-8
+    check_counts(res, ref)

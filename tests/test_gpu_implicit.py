@@ -105,3 +105,24 @@ def test_dirk_tableaus_lorenz_fp64(oracle_module, alg):
     err = normwise_rel(res.state, ref["state"])
     print(alg, "normwise rel err", err)
     assert err <= 1e-8
+
+
+@pytest.mark.parametrize("alg", ["backwards_euler", "backwards_euler_pc", "crank_nicolson"])
+def test_single_stage_implicit_lorenz_fp64(oracle_module, alg):
+    """Backward Euler (cached and predictor starts) at fixed dt 1e-3 and
+    adaptive Crank-Nicolson with its backward-Euler error estimate."""
+    n = 64
+    inits, params = lorenz_batch(n, seed=19)
+    kw = dict(dt=1e-3) if alg != "crank_nicolson" else dict(atol=1e-8, rtol=1e-6)
+    solver = ca.Solver(lorenz(np.float64), algorithm=alg, save_every=0.05,
+                       linear_correction_type="bicgstab",
+                       output_types=["state", "iteration_counters"], **kw)
+    res, ref = run_both(solver, oracle_module, "lorenz", inits, params, 1.0)
+    assert np.all(res.status_codes == 0), res.status_messages
+    err = normwise_rel(res.state, ref["state"])
+    print(alg, "normwise rel err", err)
+    assert err <= 1e-8
+    for k in (2, 3):
+        g = int(res.iteration_counters[:, k].sum())
+        o = int(ref["iteration_counters"][:, k].sum())
+        assert abs(g - o) <= 0.01 * o + 2
