@@ -52,7 +52,55 @@ def lorenz_workload(n, precision=np.float64):
                        "n_systems": n}}
 
 
-WORKLOADS = {"lorenz": lorenz_workload}
+HH_EQUATIONS = """
+alpha_m = 0.1*(V + 40)/(1 - exp(-(V + 40)/10))
+beta_m = 4*exp(-(V + 65)/18)
+alpha_h = 0.07*exp(-(V + 65)/20)
+beta_h = 1/(1 + exp(-(V + 35)/10))
+alpha_n = 0.01*(V + 55)/(1 - exp(-(V + 55)/10))
+beta_n = 0.125*exp(-(V + 65)/80)
+dV = (I_ext - gNa*m**3*h*(V - ENa) - gK*n**4*(V - EK) - gL*(V - EL))/Cm
+dm = alpha_m*(1 - m) - beta_m*m
+dh = alpha_h*(1 - h) - beta_h*h
+dn = alpha_n*(1 - n) - beta_n*n
+"""
+
+
+def hh_system(precision):
+    import cubie_amd as ca
+    return ca.create_ODE_system(
+        HH_EQUATIONS, states={"V": -65.0, "m": 0.0529, "h": 0.5961, "n": 0.3177},
+        parameters={"gNa": 120.0, "gK": 36.0, "gL": 0.3},
+        constants={"ENa": 50.0, "EK": -77.0, "EL": -54.387, "Cm": 1.0},
+        drivers=["I_ext"], precision=precision, name="hodgkin_huxley")
+
+
+def hh_workload(n, precision=np.float64, duration=500.0, dt=0.01):
+    """Hodgkin-Huxley batch: conductances uniform within +-30 % of 120/36/0.3,
+    driver = 1000 samples (0.5 ms apart) of 10 sin(2 pi t / 50) + N(0, 2),
+    linearly interpolated; L-stable SDIRK3 at fixed dt with BiCGSTAB; summary
+    outputs only (mean V, max V, spike count above 0 mV) over the whole run,
+    sampled every 0.05 ms; iteration counters at the end for the step tally."""
+    rng = np.random.default_rng(2024)
+    params = np.array([120.0, 36.0, 0.3])[:, None] * rng.uniform(0.7, 1.3, (3, n))
+    inits = np.tile(np.array([[-65.0], [0.0529], [0.5961], [0.3177]]), (1, n))
+    drng = np.random.default_rng(7)
+    t = np.arange(1000) * 0.5
+    current = 10.0 * np.sin(2.0 * np.pi * t / 50.0) + drng.normal(0.0, 2.0, 1000)
+    solver_kw = dict(algorithm="l_stable_dirk_3", dt=dt, atol=1e-6, rtol=1e-6,
+                     linear_correction_type="bicgstab", save_every=duration,
+                     summarise_every=duration, sample_summaries_every=0.05,
+                     summarise_variables=["V"], save_variables=["V"],
+                     output_types=["iteration_counters", "mean", "max", "spike_count[0]"])
+    return {"name": "hh_sdirk3", "system": hh_system(precision), "inits": inits, "params": params,
+            "duration": duration, "solver_kw": solver_kw, "oracle_system": "hodgkin_huxley",
+            "drivers": {"I_ext": current, "dt": 0.5, "order": 1, "wrap": False},
+            "config": {"workload": f"Hodgkin-Huxley L-stable SDIRK3 + BiCGSTAB, dt {dt} ms, "
+                                   f"t in [0,{duration:g}] ms, summaries mean/max/spike count of V",
+                       "n_systems": n}}
+
+
+WORKLOADS = {"lorenz": lorenz_workload, "hh": hh_workload}
 
 
 # ---------------------------------------------------------------------------

@@ -54,13 +54,14 @@ def test_hh_sdirk3_fp64(oracle_module, linear):
 
 
 def test_hh_sdirk3_fp32(oracle_module):
-    """FP32 HH at atol = rtol = 1e-5: the Newton tolerance (tol / 10) then
-    sits above single-precision resolution. At 1e-6 the oracle itself
-    reports NEWTON_DIVERGENCE for most runs."""
+    """FP32 HH at atol = rtol = 1e-4: the Newton tolerance (tol / 10) then
+    sits ~100x above single-precision resolution. At 1e-6 the oracle itself
+    reports NEWTON_DIVERGENCE for most runs, and at 1e-5 the margin is thin
+    enough that a last-bit difference can decide divergence for a run."""
     n = 64
     inits, params = hh_batch(n, seed=3)
     solver = ca.Solver(hodgkin_huxley(np.float32), algorithm="l_stable_dirk_3", dt=0.025,
-                       atol=1e-5, rtol=1e-5,
+                       atol=1e-4, rtol=1e-4,
                        linear_correction_type="bicgstab", save_every=1.0,
                        output_types=["state", "iteration_counters"])
     solver._configure_drivers(hh_driver())

@@ -1,0 +1,187 @@
+"""Host-side logic pinned against closed forms and the reference's unit
+tests: tableau order conditions, the dense predictor's ratio polynomials,
+output-event counting, driver interpolation, controller defaults, input
+grids, summary-metric bookkeeping and status decoding."""
+
+import numpy as np
+import pytest
+
+import cubie_amd as ca
+from cubie_amd.integrators import tableaus as tb
+from cubie_amd.integrators.algorithms import resolve_algorithm
+from cubie_amd.integrators.stage_predictors import (ratio_coefficients, readahead_matrix,
+                                                    sample_stages, supports_prediction)
+from cubie_amd.integrators.timing import LoopTiming
+
+RK_TABLEAUS = ([(k, v) for k, v in tb.ERK_TABLEAU_REGISTRY.items()]
+               + [(k, v) for k, v in tb.DIRK_TABLEAU_REGISTRY.items()]
+               + [(k, v) for k, v in tb.FIRK_TABLEAU_REGISTRY.items()])
+
+
+@pytest.mark.parametrize("name,tab", RK_TABLEAUS, ids=[k for k, _ in RK_TABLEAUS])
+def test_runge_kutta_order_conditions(name, tab):
+    """Sum b = 1, c = row sums of A (base_algorithm_step.py:265-300), and the
+    classical conditions up to order 3 where the tableau claims them."""
+    A = np.array(tab.a, dtype=np.float64)
+    b = np.array(tab.b, dtype=np.float64)
+    c = np.array(tab.c, dtype=np.float64)
+    assert abs(b.sum() - 1.0) < 1e-12
+    np.testing.assert_allclose(A.sum(axis=1), c, atol=1e-12)
+    if tab.order >= 2:
+        assert abs(b @ c - 0.5) < 1e-10
+    if tab.order >= 3:
+        assert abs(b @ c ** 2 - 1.0 / 3.0) < 1e-10
+        assert abs(b @ (A @ c) - 1.0 / 6.0) < 1e-10
+    if tab.b_hat is not None:
+        assert abs(np.sum(tab.b_hat) - 1.0) < 1e-10
+
+
+@pytest.mark.parametrize("name,tab", list(tb.ROSENBROCK_TABLEAUS.items()),
+                         ids=list(tb.ROSENBROCK_TABLEAUS))
+def test_rosenbrock_structure(name, tab):
+    """Stage couplings a and C are strictly lower triangular and gamma > 0.
+    (The weights act on the gamma-scaled stage variables of this W-form, so
+    they need not sum to one; accuracy is pinned by the Julia gate.)"""
+    A = np.array(tab.a, dtype=np.float64)
+    C = np.array(tab.C, dtype=np.float64)
+    assert np.all(np.triu(A) == 0.0) and np.all(np.triu(C) == 0.0)
+    assert tab.gamma > 0 and len(tab.gamma_stages) == tab.stage_count
+
+
+def test_radau_embedded_weights_reproduce_low_order_quadrature():
+    """Radau IIA(5)'s embedded weights integrate polynomials up to the
+    embedded order exactly (the Vandermonde construction,
+    generic_firk_tableaus.py:77-118)."""
+    tab = tb.RADAU_IIA_5
+    c = np.array(tab.c)
+    bh = np.array(tab.b_hat)
+    for k in range(2):
+        assert abs(bh @ c ** k - 1.0 / (k + 1)) < 1e-12
+
+
+@pytest.mark.parametrize("tab", [tb.RADAU_IIA_5, tb.GAUSS_LEGENDRE_2, tb.KVAERNO5, tb.L_STABLE_DIRK3],
+                         ids=lambda t: t.name)
+@pytest.mark.parametrize("ratio", [0.5, 1.0, 1.7])
+def test_predictor_ratio_polynomials_equal_readahead(tab, ratio):
+    """The stored per-power coefficients reproduce the Lagrange read-ahead
+    matrix at any step ratio (so the device Horner evaluation is exact)."""
+    if not supports_prediction(tab):
+        pytest.skip("tableau does not predict")
+    coeffs = ratio_coefficients(tab)
+    poly = sum(coeffs[p] * ratio ** (p + 1) for p in range(coeffs.shape[0]))
+    direct = readahead_matrix(np.array(tab.c, dtype=np.float64), ratio, sample_stages(tab.c))
+    np.testing.assert_allclose(poly, direct, atol=1e-10)
+
+
+def test_lorenz_output_rows():
+    """t in [0, 10] saved every 0.01: 1000 regular events plus the initial
+    save (SingleIntegratorRun.py:153-325)."""
+    t = LoopTiming.resolve(np.float64, save_every=0.01, summarise_every=None,
+                           sample_summaries_every=None, has_time_domain=True, has_summaries=False)
+    assert t.output_length(10.0) == 1001
+    assert t.regular_event_count(10.0, 0.01) == 1000
+
+
+def test_event_count_allowance():
+    """Floating-point noise in duration / interval is forgiven up to 4 eps
+    per event, never more than half an interval."""
+    t = LoopTiming.resolve(np.float64, save_every=0.1, summarise_every=None,
+                           sample_summaries_every=None, has_time_domain=True, has_summaries=False)
+    assert t.regular_event_count(0.3, 0.1) == 3          # 0.3/0.1 = 2.9999999999999996
+    assert t.regular_event_count(0.35, 0.1) == 3
+
+
+def test_linear_driver_closed_form():
+    """Order-1 interpolation: each segment is y_k + (y_{k+1} - y_k) tau
+    (array_interpolator.py:960-972); evaluation hits the samples."""
+    y = np.array([0.0, 2.0, -1.0, 4.0, 0.0])     # periodic: last sample equals the first
+    it = ca.ArrayInterpolator(np.float64, {"d": y, "dt": 0.5, "order": 1, "wrap": True})
+    co = np.asarray(it.coefficients)
+    for k in range(3):
+        np.testing.assert_allclose(co[k, 0], [y[k], y[k + 1] - y[k]])
+    np.testing.assert_allclose(it.evaluate(np.array([0.0, 0.5, 1.0, 1.25]))[:, 0],
+                               [0.0, 2.0, -1.0, 1.5])
+
+
+@pytest.mark.parametrize("alias,kind", [("dormand-prince-54", "pi"), ("radau", "gustafsson"),
+                                        ("ros3p", "gustafsson"), ("kvaerno5", "gustafsson"),
+                                        ("l_stable_dirk_3", "fixed"), ("rk4", "fixed")])
+def test_controller_defaults_per_family(alias, kind):
+    """generic_erk.py:77-88, generic_dirk.py:69-104, generic_firk.py:71-104,
+    generic_rosenbrock_w.py:71-106."""
+    sysm = ca.create_ODE_system(["dx = -x"], states={"x": 1.0}, precision=np.float64, name="d1")
+    solver = ca.Solver(sysm, algorithm=alias, save_every=0.1, output_types=["state"])
+    c = solver.kernel.controller
+    assert c.kind == kind
+    if kind == "pi":
+        assert (c.kp, c.ki, c.min_gain, c.max_gain, c.safety) == (0.7, -0.4, 0.2, 10.0, 0.9)
+    if kind == "gustafsson":
+        assert (c.min_gain, c.max_gain, c.safety, c.deadband_min, c.deadband_max) == (0.2, 8.0, 0.9, 1.0, 1.2)
+
+
+def test_adaptive_dt_bounds_derivation():
+    """dt given: dt_min = dt / 100, dt_max = dt * 100
+    (adaptive_step_controller.py:44-230)."""
+    sysm = ca.create_ODE_system(["dx = -x"], states={"x": 1.0}, precision=np.float64, name="d2")
+    c = ca.Solver(sysm, algorithm="dopri54", dt=1e-3, save_every=0.1,
+                  output_types=["state"]).kernel.controller
+    assert np.isclose(c.dt_min, 1e-5) and np.isclose(c.dt_max, 1e-1)
+
+
+def test_inner_tolerances_follow_controller():
+    """Newton = controller tolerances / 10, Krylov = controller tolerances,
+    residual reduction = min rtol (/100 for Rosenbrock)
+    (SingleIntegratorRunCore.py:353-414)."""
+    sysm = ca.create_ODE_system(["dx = -x", "dy = -2*y"], states={"x": 1.0, "y": 1.0},
+                                precision=np.float64, name="d3")
+    k = ca.Solver(sysm, algorithm="kvaerno3", atol=[1e-8, 1e-9], rtol=1e-5,
+                  save_every=0.1, output_types=["state"]).kernel
+    im = k.implicit
+    np.testing.assert_allclose(im.tol("newton_atol", 2), [1e-9, 1e-10])
+    np.testing.assert_allclose(im.tol("krylov_rtol", 2), [1e-5, 1e-5])
+    assert im.krylov_residual_reduction == pytest.approx(1e-5)
+    kr = ca.Solver(sysm, algorithm="ros3p", atol=1e-8, rtol=1e-5, save_every=0.1,
+                   output_types=["state"]).kernel
+    assert kr.implicit.krylov_residual_reduction == pytest.approx(1e-7)
+
+
+def test_combinatorial_and_verbatim_grids():
+    sysm = ca.create_ODE_system(["dx = -a*x", "dy = -b*y"], states={"x": 1.0, "y": 2.0},
+                                parameters={"a": 1.0, "b": 2.0}, precision=np.float64, name="g")
+    solver = ca.Solver(sysm, algorithm="rk4", dt=0.1, save_every=0.1, output_types=["state"])
+    inits, params = solver.build_grid({"x": [1.0, 2.0, 3.0]}, {"a": [0.1, 0.2]}, grid_type="combinatorial")
+    assert inits.shape == (2, 6) and params.shape == (2, 6)
+    inits, params = solver.build_grid({"x": [1.0, 2.0]}, {"a": [0.1, 0.2]}, grid_type="verbatim")
+    assert inits.shape == (2, 2)
+    np.testing.assert_allclose(params[0], [0.1, 0.2])
+    np.testing.assert_allclose(params[1], [2.0, 2.0])
+
+
+def test_summary_metric_layout():
+    from cubie_amd.outputhandling.summary_metrics import summary_metrics as sm
+    req = sm.parse(["mean", "max", "peaks[3]", "spike_count[0]"])
+    names = [m for m, _ in req]
+    assert names == ["mean", "max", "peaks", "spike_count"]
+    assert sm["peaks"].output_len(3) == 3
+    assert sm.legend(["mean", "peaks[2]"])[:3] == ["mean", "peaks_1", "peaks_2"]
+
+
+def test_status_decoding():
+    from cubie_amd.result_codes import CUBIE_RESULT_CODES, decode_status_codes
+    codes = np.array([0, 6, 256 | 8], dtype=np.int32)
+    out = decode_status_codes(codes)
+    assert 0 not in out
+    assert out[1] == ["MAX_NEWTON_ITERATIONS_EXCEEDED", "MAX_LINEAR_ITERATIONS_EXCEEDED"]
+    assert set(out[2]) == {"STEP_TOO_SMALL", "NEWTON_DIVERGENCE"}
+    assert CUBIE_RESULT_CODES.DT_EFF_EFFECTIVELY_ZERO == 16
+
+
+def test_algorithm_aliases_resolve_to_families():
+    assert resolve_algorithm("ode45").family == "erk"
+    assert resolve_algorithm("radau").family == "firk"
+    assert resolve_algorithm("ode23s").family == "rosenbrock"
+    assert resolve_algorithm("crank_nicolson").family == "crank_nicolson"
+    assert resolve_algorithm("backwards_euler_pc").family == "backwards_euler_pc"
+    assert resolve_algorithm(tb.KVAERNO5).family == "dirk"
+    with pytest.raises(ValueError):
+        resolve_algorithm("not-a-method")
