@@ -4,10 +4,16 @@ One "step" of this benchmark is one launch of the integration kernel over
 the whole synthetic batch (every system from t0 to t_end), inputs already
 resident in HBM. The timed region is K launches bracketed by
 torch.cuda.synchronize(); each launch is also timed with a pair of hipEvents
-recorded on the launch stream (the roofline uses their mean).
+recorded on the launch stream (value and roofline use their mean).
 
 value = total accepted steps (sum over systems of steps - rejections, read
-from the kernel's own iteration counters) / mean kernel time.
+from the kernel's own iteration counters) / mean kernel time, for the
+headline kernel variant (the first requested variant whose runs all
+succeed); every variant's rate, its speed-up over the FP64 plain kernel and
+its summary error against it are reported under config.variants.
+
+Default workload: the 1,048,576-system Hodgkin-Huxley batch (L-stable
+SDIRK3 + BiCGSTAB at dt 0.01 ms over 500 ms, summaries only).
 
 Prints one JSON line (see the repository's benchmark contract).
 """
@@ -34,30 +40,38 @@ HBM_PEAK = 8000.0                    # GB/s
 # Workloads
 # ---------------------------------------------------------------------------
 
-def lorenz_workload(n, precision=np.float64):
+def lorenz_system(precision):
     import cubie_amd as ca
-    sysm = ca.create_ODE_system(
+    return ca.create_ODE_system(
         ["dx = sigma*(y - x)", "dy = x*(rho - z) - y", "dz = x*y - beta*z"],
         states={"x": 1.0, "y": 0.0, "z": 25.0},
         parameters={"sigma": 10.0, "rho": 28.0, "beta": 8.0 / 3.0},
         precision=precision, name="lorenz")
+
+
+def lorenz_workload(n, precision=np.float64, duration=10.0):
+    sysm = lorenz_system(precision)
     rng = np.random.default_rng(1234)
     params = np.array([10.0, 28.0, 8.0 / 3.0])[:, None] * rng.uniform(0.9, 1.1, (3, n))
     inits = rng.uniform(-1, 1, (3, n)) + np.array([0.0, 0.0, 25.0])[:, None]
     solver_kw = dict(algorithm="dormand-prince-54", atol=1e-8, rtol=1e-6, save_every=0.01,
                      output_types=["state", "iteration_counters"])
-    return {"name": "lorenz_dopri5", "system": sysm, "inits": inits, "params": params,
-            "duration": 10.0, "solver_kw": solver_kw, "oracle_system": "lorenz",
+    return {"name": "lorenz_dopri5", "system": sysm, "system_factory": lorenz_system,
+            "inits": inits, "params": params,
+            "duration": duration, "solver_kw": solver_kw, "oracle_system": "lorenz",
             "config": {"workload": "Lorenz DOPRI5 rtol 1e-6 atol 1e-8, t in [0,10], save every 0.01",
                        "n_systems": n}}
 
 
+# alpha_m and alpha_n have removable singularities at V = -40 / -55 mV; within
+# 1e-3 mV of them the first-order expansion of x / (1 - exp(-x/10)) = 10 + x/2
+# keeps every evaluation finite (an FP32 stage state can land exactly on them).
 HH_EQUATIONS = """
-alpha_m = 0.1*(V + 40)/(1 - exp(-(V + 40)/10))
+alpha_m = Piecewise((1 + (V + 40)/20, Abs(V + 40) < 1e-3), (0.1*(V + 40)/(1 - exp(-(V + 40)/10)), True))
 beta_m = 4*exp(-(V + 65)/18)
 alpha_h = 0.07*exp(-(V + 65)/20)
 beta_h = 1/(1 + exp(-(V + 35)/10))
-alpha_n = 0.01*(V + 55)/(1 - exp(-(V + 55)/10))
+alpha_n = Piecewise((0.1 + (V + 55)/200, Abs(V + 55) < 1e-3), (0.01*(V + 55)/(1 - exp(-(V + 55)/10)), True))
 beta_n = 0.125*exp(-(V + 65)/80)
 dV = (I_ext - gNa*m**3*h*(V - ENa) - gK*n**4*(V - EK) - gL*(V - EL))/Cm
 dm = alpha_m*(1 - m) - beta_m*m
@@ -75,12 +89,17 @@ def hh_system(precision):
         drivers=["I_ext"], precision=precision, name="hodgkin_huxley")
 
 
-def hh_workload(n, precision=np.float64, duration=500.0, dt=0.01):
+def hh_workload(n, precision=np.float64, duration=500.0, dt=2.0 ** -6):
     """Hodgkin-Huxley batch: conductances uniform within +-30 % of 120/36/0.3,
     driver = 1000 samples (0.5 ms apart) of 10 sin(2 pi t / 50) + N(0, 2),
     linearly interpolated; L-stable SDIRK3 at fixed dt with BiCGSTAB; summary
     outputs only (mean V, max V, spike count above 0 mV) over the whole run,
-    sampled every 0.05 ms; iteration counters at the end for the step tally."""
+    sampled every 4 steps; iteration counters at the end for the step tally.
+
+    dt = 2^-6 ms and the sampling interval 2^-4 ms are binary fractions, so
+    step ends land exactly on the sampling events; with a decimal dt such as
+    0.01 the accumulated time drifts against the event schedule and the
+    loop's positive-gap clamp inserts ~0.8 extra micro-steps per event."""
     rng = np.random.default_rng(2024)
     params = np.array([120.0, 36.0, 0.3])[:, None] * rng.uniform(0.7, 1.3, (3, n))
     inits = np.tile(np.array([[-65.0], [0.0529], [0.5961], [0.3177]]), (1, n))
@@ -89,13 +108,14 @@ def hh_workload(n, precision=np.float64, duration=500.0, dt=0.01):
     current = 10.0 * np.sin(2.0 * np.pi * t / 50.0) + drng.normal(0.0, 2.0, 1000)
     solver_kw = dict(algorithm="l_stable_dirk_3", dt=dt, atol=1e-6, rtol=1e-6,
                      linear_correction_type="bicgstab", save_every=duration,
-                     summarise_every=duration, sample_summaries_every=0.05,
+                     summarise_every=duration, sample_summaries_every=4 * dt,
                      summarise_variables=["V"], save_variables=["V"],
                      output_types=["iteration_counters", "mean", "max", "spike_count[0]"])
-    return {"name": "hh_sdirk3", "system": hh_system(precision), "inits": inits, "params": params,
+    return {"name": "hh_sdirk3", "system": hh_system(precision), "system_factory": hh_system,
+            "inits": inits, "params": params,
             "duration": duration, "solver_kw": solver_kw, "oracle_system": "hodgkin_huxley",
             "drivers": {"I_ext": current, "dt": 0.5, "order": 1, "wrap": False},
-            "config": {"workload": f"Hodgkin-Huxley L-stable SDIRK3 + BiCGSTAB, dt {dt} ms, "
+            "config": {"workload": f"Hodgkin-Huxley L-stable SDIRK3 + BiCGSTAB, dt {dt:g} ms, "
                                    f"t in [0,{duration:g}] ms, summaries mean/max/spike count of V",
                        "n_systems": n}}
 
@@ -107,14 +127,18 @@ WORKLOADS = {"lorenz": lorenz_workload, "hh": hh_workload}
 # Flop model (algorithmic work per attempted step)
 # ---------------------------------------------------------------------------
 
-def flops_per_attempted_step(kernel, spec):
-    """Algorithmic flops of one attempted step (add/mul/div = 1, FMA = 2,
-    transcendental = 1), from the generated system's IR counts and the
-    tableau's nonzero structure."""
+def _f(ops, key):
+    c = ops.get(key, {"flops": 0, "transcendental": 0})
+    return c["flops"] + c["transcendental"]
+
+
+def erk_flops_per_attempted_step(kernel, spec):
+    """Algorithmic flops of one attempted explicit step (add/mul/div = 1,
+    FMA = 2, transcendental = 1), from the generated system's IR counts and
+    the tableau's nonzero structure."""
     tab = kernel.algorithm.tableau
     n = kernel.system.num_states
-    ops = spec.op_counts
-    f_rhs = ops["dxdt"]["flops"] + ops["dxdt"]["transcendental"]
+    f_rhs = _f(spec.op_counts, "dxdt")
     s = tab.stage_count
     rhs_evals = s - (1 if tab.first_same_as_last else 0)
     nnz_a = sum(1 for i in range(s) for j in range(i) if tab.a[i][j] != 0.0)
@@ -122,115 +146,215 @@ def flops_per_attempted_step(kernel, spec):
     sol = 0 if tab.b_matches_a_row is not None else 2 * n * sum(1 for b in tab.b if b) + 2 * n
     err = 0
     if kernel.algorithm.is_adaptive:
-        d = tab.d
-        err = 2 * n * sum(1 for x in d if x) + n
+        err = 2 * n * sum(1 for x in tab.d if x) + n
     norm = 7 * n + 2                                       # |e|, max, tol, ratio, square, sum
     ctrl = 12                                             # 2 pow + clamps + products
     return rhs_evals * f_rhs + stage_forms + sol + err + norm + ctrl
 
 
+def implicit_flops(kernel, spec, work, steps):
+    """Algorithmic flops of an implicit solve from the oracle's exact tally
+    of right-hand sides, Jacobian-vector products, time derivatives, Krylov
+    and Newton iterations (the GPU follows the same iteration counts):
+    F = rhs F_f + jvp F_jvp + dfdt F_dfdt + K F_iter(W) + N F_newton(W)
+        + steps F_step,
+    with the vector work of one BiCGSTAB iteration 36 W + 4 order W, of one
+    MR/SD iteration 14 W + 2 order W, of one Newton iteration 8 W, and 10 n s
+    per step for stage combinations, the error norm and the controller."""
+    n = kernel.system.num_states
+    tab = kernel.algorithm.tableau
+    s = tab.stage_count
+    fam = kernel.algorithm.family
+    W = s * n if fam == "firk" else n
+    order = kernel.implicit.preconditioner_order
+    bicg = kernel.implicit.linear_correction_type == "bicgstab"
+    f_iter = (36 * W + 4 * order * W) if bicg else (14 * W + 2 * order * W)
+    ops = spec.op_counts
+    tot = work.sum(axis=0).astype(np.float64)
+    flops = (tot[0] * _f(ops, "dxdt") + tot[1] * _f(ops, "jvp") + tot[3] * _f(ops, "dfdt")
+             + tot[4] * f_iter + tot[5] * 8 * W + steps * 10 * n * s)
+    return flops / max(steps, 1)
+
+
+# ---------------------------------------------------------------------------
+# CPU leg: the oracle (C++ restatement of the reference's algorithms)
 # ---------------------------------------------------------------------------
 
-def cpu_baseline(work, solver, seconds_target=15.0):
-    """Oracle (CPU restatement) on a bounded sample, all host threads."""
+def run_oracle(work, kernel, n_sample, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     orc.build()
-    threads = os.cpu_count() or 1
-    run = orc.OracleRun(solver.kernel, work["oracle_system"], work["duration"])
-    n_total = work["inits"].shape[1]
-    sample = min(n_total, 256 * threads)
+    run = orc.OracleRun(kernel, work["oracle_system"], work["duration"])
     t = time.perf_counter()
-    res = run.solve(work["inits"][:, :sample], work["params"][:, :sample], threads=threads)
-    el = time.perf_counter() - t
-    # grow the sample once to reach a 10-30 s measurement when the first is short
+    res = run.solve(work["inits"][:, :n_sample], work["params"][:, :n_sample], threads=threads)
+    return res, time.perf_counter() - t
+
+
+def cpu_baseline(work, kernel, seconds_target=15.0):
+    """Oracle on a bounded sample of the same workload, all host threads."""
+    threads = os.cpu_count() or 1
+    n_total = work["inits"].shape[1]
+    sample = min(n_total, 16 * threads)
+    res, el = run_oracle(work, kernel, sample, threads)
+    # grow the sample once to reach a 10-30 s measurement
     if el < seconds_target / 4 and sample < n_total:
-        sample = int(min(n_total, sample * max(2, seconds_target / max(el, 1e-3))))
-        t = time.perf_counter()
-        res = run.solve(work["inits"][:, :sample], work["params"][:, :sample], threads=threads)
-        el = time.perf_counter() - t
+        sample = int(min(n_total, sample * max(2.0, seconds_target / max(el, 1e-3))))
+        res, el = run_oracle(work, kernel, sample, threads)
     ic = res["iteration_counters"]
     acc = int((ic[:, 2, :].astype(np.int64) - ic[:, 3, :]).sum())
-    return {"value": acc / el, "unit": "accepted steps/s", "cores": threads, "kind": "port",
-            "sample": f"{sample} of {n_total} systems, {el:.1f} s on {threads} host threads"}
+    steps = int(ic[:, 2, :].astype(np.int64).sum())
+    return ({"value": acc / el, "unit": "accepted steps/s", "cores": threads, "kind": "port",
+             "sample": f"{sample} of {n_total} systems ({acc} accepted steps), {el:.1f} s on "
+                       f"{threads} host threads"},
+            res, steps)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lorenz", choices=sorted(WORKLOADS))
-    ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--compile-only", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(systems are independent: a multi-GPU run would shard the batch with no exchange).",
-              file=sys.stderr)
-        sys.exit(2)
+# ---------------------------------------------------------------------------
+# GPU leg
+# ---------------------------------------------------------------------------
 
+VARIANTS = {
+    # name: (precision, kernel_mode, precision_mode)
+    "fp64_plain": (np.float64, "plain", "plain"),
+    "fp32_plain": (np.float32, "plain", "plain"),
+    "fp32_compensated_persistent": (np.float32, "persistent", "compensated"),
+}
+DEFAULT_VARIANTS = {"hh": ["fp64_plain"], "lorenz": ["fp64_plain"]}
+
+
+def make_solver(work, precision, kernel_mode, precision_mode):
     import cubie_amd as ca
-    work = WORKLOADS[args.workload](args.n)
-    solver = ca.Solver(work["system"], **work["solver_kw"])
-    if args.compile_only:
-        solver.kernel.compile_only(work["duration"])
-        return
+    w = dict(work)
+    sysm = w["system_factory"](precision) if "system_factory" in w else w["system"]
+    solver = ca.Solver(sysm, kernel_mode=kernel_mode, precision_mode=precision_mode,
+                       **w["solver_kw"])
+    if w.get("drivers") is not None:
+        solver._configure_drivers(w["drivers"])
+    return solver
 
+
+def measure(work, solver, steps, warmup):
     import torch
     kernel = solver.kernel
-    inits = torch.from_numpy(work["inits"].astype(kernel.precision)).cuda()
-    params = torch.from_numpy(work["params"].astype(kernel.precision)).cuda()
-    dtype = "f64" if kernel.precision is np.float64 else "f32"
-
-    def launch():
-        return kernel.run(inits, params, duration=work["duration"], timed=True)
-
-    for _ in range(args.warmup):
-        out, ms, info = launch()
+    prec = kernel.precision
+    inits = torch.from_numpy(work["inits"].astype(prec)).cuda()
+    params = torch.from_numpy(work["params"].astype(prec)).cuda()
+    for _ in range(warmup):
+        out, ms, info = kernel.run(inits, params, duration=work["duration"], timed=True)
     torch.cuda.synchronize()
     times = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out, ms, info = launch()
+    for _ in range(steps):
+        out, ms, info = kernel.run(inits, params, duration=work["duration"], timed=True)
         times.append(ms)
+        print(f"# {kernel.precision.__name__} {kernel.kernel_mode}/{kernel.precision_mode}: "
+              f"{ms:.1f} ms", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ic = out.iteration_counters.cpu().numpy()
     steps_total = int(ic[:, 2, :].astype(np.int64).sum())
     accepted = int((ic[:, 2, :].astype(np.int64) - ic[:, 3, :]).sum())
     status = out.status_codes.cpu().numpy()
+    summ = out.state_summaries.cpu().numpy() if out.state_summaries.numel() > 1 else None
     mean_ms = float(np.mean(times))
-    value = accepted / (mean_ms * 1e-3)
-    spec = info["spec"]
-    fps = flops_per_attempted_step(kernel, spec)
-    achieved = steps_total * fps / (mean_ms * 1e-3) / 1e12
+    return {"ms": mean_ms, "wall_ms": wall / steps * 1e3, "steps": steps_total,
+            "accepted": accepted, "value": accepted / (mean_ms * 1e-3),
+            "failed_runs": int((status != 0).sum()), "summaries": summ,
+            "spec": info["spec"], "kernel_info": info["module"].info()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="hh", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--duration", type=float, default=None)
+    ap.add_argument("--variants", default=None,
+                    help="comma-separated subset of " + ",".join(VARIANTS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compile-only", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="instead of the benchmark, run a plain device copy of this many "
+                         "bytes (the reference kernel for scaling HBM byte counters)")
+    args = ap.parse_args()
+    if args.copy_bytes:
+        import torch
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "launches": 3}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported here "
+              "(systems are independent, so a multi-GPU run would shard the batch along "
+              "the run axis with no exchange).", file=sys.stderr)
+        sys.exit(2)
+
+    kw = {} if args.duration is None else {"duration": args.duration}
+    work = WORKLOADS[args.workload](args.n, **kw)
+    names = (args.variants.split(",") if args.variants
+             else DEFAULT_VARIANTS.get(args.workload, ["fp64_plain"]))
+    solvers = {nm: make_solver(work, *VARIANTS[nm]) for nm in names}
+    if args.compile_only:
+        for sv in solvers.values():
+            sv.kernel.compile_only(work["duration"])
+        return
+
+    results = {nm: measure(work, sv, args.steps, args.warmup) for nm, sv in solvers.items()}
+    # headline: the first listed variant whose runs all succeeded
+    head = next((nm for nm in names if results[nm]["failed_runs"] == 0), names[0])
+    r = results[head]
+    kernel = solvers[head].kernel
+    dtype = "f64" if kernel.precision is np.float64 else "f32"
+
+    # accuracy of every variant against the FP64 kernel's summaries
+    ref = results.get("fp64_plain", {}).get("summaries")
+    variants_out = {}
+    for nm, res in results.items():
+        entry = {k: res[k] for k in ("value", "ms", "accepted", "failed_runs", "kernel_info")}
+        if ref is not None and res["summaries"] is not None and nm != "fp64_plain":
+            a, b = res["summaries"].astype(np.float64), ref
+            entry["max_rel_err_vs_fp64"] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+        if "fp64_plain" in results:
+            entry["speedup_vs_fp64_plain"] = res["value"] / results["fp64_plain"]["value"]
+        variants_out[nm] = entry
+
+    cpu, orc_res, orc_steps = (None, None, 0)
+    if not args.no_cpu:
+        cpu, orc_res, orc_steps = cpu_baseline(work, kernel)
+    if kernel.algorithm.family == "erk":
+        fps = erk_flops_per_attempted_step(kernel, r["spec"])
+    elif orc_res is not None:
+        fps = implicit_flops(kernel, r["spec"], orc_res["work"], orc_steps)
+    else:
+        fps = None
+    achieved = r["steps"] * fps / (r["ms"] * 1e-3) / 1e12 if fps else None
     line = {
         "metric": "accepted steps/s",
-        "value": value,
+        "value": r["value"],
         "unit": "accepted steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mean_ms,
-        "wall_ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic",
-        "config": dict(work["config"], kernel_mode=kernel.kernel_mode,
-                       precision_mode=kernel.precision_mode,
-                       failed_runs=int((status != 0).sum()),
-                       attempted_steps=steps_total, accepted_steps=accepted,
-                       kernel_info=info["module"].info()),
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[dtype],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[dtype], "traffic": None,
+        "config": dict(work["config"], headline=head, kernel_mode=kernel.kernel_mode,
+                       precision_mode=kernel.precision_mode, attempted_steps=r["steps"],
+                       accepted_steps=r["accepted"], variants=variants_out),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[dtype], "unit": "TFLOP/s",
+                     "frac": (achieved / PEAK[dtype]) if achieved else None, "traffic": None,
                      "flops_per_attempted_step": fps},
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(work, solver)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line))
 
 

@@ -230,11 +230,15 @@ struct DIRKStep {
     __device__ __forceinline__ int solve_stage(Ctx& ctx, const Vec<real, n>& base, const Vec<real, n>& state,
                                                real t, real h, real a, int* cnt) {
         DirkStageProblem<Cfg, Sys> P;
+        LinInputs<Cfg> li;
+        li.bind(ctx.params.data(), ctx.proposed_drivers.data());
         P.u = stage_increment.data();
         P.base = base.data();
         P.step_start = state.data();
         P.params = ctx.params.data();
         P.drivers = ctx.proposed_drivers.data();
+        P.params_l = li.params;
+        P.drivers_l = li.drivers;
         P.t = t;
         P.h = h;
         P.a = a;

@@ -13,7 +13,8 @@ namespace cubie {
 template <typename Cfg>
 struct Drivers {
     using real = typename Cfg::real;
-    const real* __restrict__ coeffs;
+    using io_real = typename Cfg::io_real;
+    const io_real* __restrict__ coeffs;   // stored in the system precision
 
     __device__ __forceinline__ void locate(real time, int& seg, real& tau, bool& in_range) const {
         constexpr real inv_res = Cfg::drv_inv_resolution;
@@ -42,12 +43,12 @@ struct Drivers {
             bool in_range;
             locate(time, seg, tau, in_range);
             constexpr int K = Cfg::drv_order + 1;
-            const real* row = coeffs + (long long)seg * (Cfg::n_drivers * K);
+            const io_real* row = coeffs + (long long)seg * (Cfg::n_drivers * K);
 #pragma unroll
             for (int i = 0; i < Cfg::n_drivers; ++i) {
                 real acc = real(0);
 #pragma unroll
-                for (int k = Cfg::drv_order; k >= 0; --k) acc = acc * tau + __ldg(&row[i * K + k]);
+                for (int k = Cfg::drv_order; k >= 0; --k) acc = acc * tau + real(__ldg(&row[i * K + k]));
                 out[i] = in_range ? acc : real(0);
             }
         }
@@ -61,13 +62,13 @@ struct Drivers {
             bool in_range;
             locate(time, seg, tau, in_range);
             constexpr int K = Cfg::drv_order + 1;
-            const real* row = coeffs + (long long)seg * (Cfg::n_drivers * K);
+            const io_real* row = coeffs + (long long)seg * (Cfg::n_drivers * K);
 #pragma unroll
             for (int i = 0; i < Cfg::n_drivers; ++i) {
                 real acc = real(0);
 #pragma unroll
                 for (int k = Cfg::drv_order; k > 0; --k)
-                    acc = acc * tau + real(k) * __ldg(&row[i * K + k]);
+                    acc = acc * tau + real(k) * real(__ldg(&row[i * K + k]));
                 out[i] = in_range ? acc * Cfg::drv_inv_resolution : real(0);
             }
         }

@@ -8,7 +8,8 @@
 // tableau is a compile-time constant, so zero coefficients cost nothing and
 // every stage vector lives in VGPRs. First-same-as-last reuse is taken only
 // when every active lane of the wave accepted its previous step, as the
-// reference does with its warp vote.
+// reference does with its warp vote. In the compensated mode the step also
+// exposes its unrounded increment for the loop's Kahan state update.
 #pragma once
 
 namespace cubie {
@@ -19,6 +20,9 @@ struct ERKStep {
     static constexpr int n = Cfg::n;
     static constexpr int s = Cfg::stages;
     Vec<real, n> stage_rhs;   // persistent: FSAL cache
+    // In the compensated mode: the step's increment y_{n+1} - y_n before it
+    // is rounded into y, for the loop's Kahan-compensated state update.
+    Vec<real, (Cfg::kahan_state ? n : 1)> increment;
 
     __device__ __forceinline__ void init() {
 #pragma unroll
@@ -65,6 +69,12 @@ struct ERKStep {
             }
             const real stage_time = current_time + dt * Cfg::c[stage_idx];
             // Stage state y_j = (sum_i a_ji k_i) * dt + y
+            if constexpr (Cfg::kahan_state && Cfg::b_row >= 0) {
+                if (prev == Cfg::b_row - 1) {
+#pragma unroll
+                    for (int i = 0; i < n; ++i) increment[i] = acc[prev * n + i] * dt;
+                }
+            }
 #pragma unroll
             for (int i = 0; i < n; ++i) acc[prev * n + i] = acc[prev * n + i] * dt + state[i];
             ctx.drv.eval(stage_time, ctx.proposed_drivers.data());
@@ -93,6 +103,7 @@ struct ERKStep {
         }
 #pragma unroll
         for (int i = 0; i < n; ++i) {
+            if constexpr (Cfg::kahan_state && Cfg::accumulates_output) increment[i] = proposed[i] * dt;
             if constexpr (Cfg::accumulates_output) proposed[i] = proposed[i] * dt + state[i];
             if constexpr (Cfg::has_error) {
                 if constexpr (Cfg::accumulates_error) error[i] *= dt;

@@ -18,11 +18,15 @@ template <typename Cfg, typename Sys, typename Ctx>
 __device__ __forceinline__ int single_stage_solve(Ctx& ctx, real* u, const real* base, const real* state,
                                                   real t, real h, real a, real& theta, int* counters) {
     DirkStageProblem<Cfg, Sys> P;
+    LinInputs<Cfg> li;
+    li.bind(ctx.params.data(), ctx.proposed_drivers.data());
     P.u = u;
     P.base = base;
     P.step_start = state;
     P.params = ctx.params.data();
     P.drivers = ctx.proposed_drivers.data();
+    P.params_l = li.params;
+    P.drivers_l = li.drivers;
     P.t = t;
     P.h = h;
     P.a = a;

@@ -21,6 +21,7 @@ namespace cubie {
 
 template <typename Cfg, typename Sys>
 struct FirkProblem {
+    using L = typename Cfg::lin_real;
     static constexpr int n = Cfg::n;
     static constexpr int s = Cfg::stages;
     static constexpr int W = s * n;
@@ -29,58 +30,68 @@ struct FirkProblem {
     const real* base;        // y_n
     const real* params;
     const real* drivers;     // s stacked driver vectors
+    const L* params_l;       // the same two, in the linear-solve precision
+    const L* drivers_l;
     real t, h;
 
     __device__ __forceinline__ real stage_time(int i) const { return t + h * Cfg::c[i]; }
-    __device__ __forceinline__ const real* stage_drivers(int i) const { return drivers + i * nd; }
 
     // out = (with_base ? y_n : 0) + sum_j a_ij vec_j, skipping zero couplings.
-    template <bool with_base>
-    __device__ __forceinline__ void combine(int i, const real* vec, real* out) const {
+    template <bool with_base, typename T, typename V>
+    __device__ __forceinline__ void combine(int i, const V* vec, T* out) const {
 #pragma unroll
         for (int k = 0; k < n; ++k) {
-            real acc = with_base ? base[k] : real(0);
+            T acc = with_base ? T(base[k]) : T(0);
 #pragma unroll
             for (int j = 0; j < s; ++j)
-                if (Cfg::A[i][j] != 0.0) acc += Cfg::A[i][j] * vec[j * n + k];
+                if (Cfg::A[i][j] != 0.0) acc += T(Cfg::A[i][j]) * T(vec[j * n + k]);
             out[k] = acc;
         }
     }
+    // Stage state in the linear-solve precision (rounded once from `real`).
+    __device__ __forceinline__ void stage_state_l(int i, L* Y) const {
+        real Yr[n];
+        combine<true>(i, u, Yr);
+#pragma unroll
+        for (int k = 0; k < n; ++k) Y[k] = L(Yr[k]);
+    }
 
-    __device__ __forceinline__ void op(const real* v, real* out) const {
+    __device__ __forceinline__ void op(const L* v, L* out) const {
 #pragma unroll
         for (int i = 0; i < s; ++i) {
-            real Y[n], dir[n], jv[n];
-            combine<true>(i, u, Y);
+            L Y[n], dir[n], jv[n];
+            stage_state_l(i, Y);
             combine<false>(i, v, dir);
-            Sys::jvp(Y, params, stage_drivers(i), dir, jv, stage_time(i));
+            Sys::jvp(Y, params_l, drivers_l + i * nd, dir, jv, L(stage_time(i)));
+            const L hl = L(h);
 #pragma unroll
-            for (int k = 0; k < n; ++k) out[i * n + k] = v[i * n + k] - h * jv[k];
+            for (int k = 0; k < n; ++k) out[i * n + k] = v[i * n + k] - hl * jv[k];
         }
     }
-    __device__ __forceinline__ void neumann(const real* v, real* out) const {
+    __device__ __forceinline__ void neumann(const L* v, L* out) const {
+        const L hl = L(h);
 #pragma unroll
         for (int e = 0; e < W; ++e) out[e] = v[e];
         for (int it = 0; it < Cfg::preconditioner_order; ++it) {
-            real jvp[W];
+            L jvp[W];
 #pragma unroll
             for (int i = 0; i < s; ++i) {
-                real Y[n], dir[n];
-                combine<true>(i, u, Y);
+                L Y[n], dir[n];
+                stage_state_l(i, Y);
                 combine<false>(i, out, dir);
-                Sys::jvp(Y, params, stage_drivers(i), dir, jvp + i * n, stage_time(i));
+                Sys::jvp(Y, params_l, drivers_l + i * nd, dir, jvp + i * n, L(stage_time(i)));
             }
 #pragma unroll
-            for (int e = 0; e < W; ++e) out[e] = v[e] + h * jvp[e];
+            for (int e = 0; e < W; ++e) out[e] = v[e] + hl * jvp[e];
         }
     }
-    __device__ __forceinline__ void jacobi(const real* v, real* out) const {
+    __device__ __forceinline__ void jacobi(const L* v, L* out) const {
 #pragma unroll
         for (int i = 0; i < s; ++i) {
-            real Y[n], jd[n];
-            combine<true>(i, u, Y);
-            Sys::jac_diag(Y, params, stage_drivers(i), jd, stage_time(i));
-            const real scale = h * Cfg::A[i][i];
+            L Y[n], jd[n];
+            stage_state_l(i, Y);
+            Sys::jac_diag(Y, params_l, drivers_l + i * nd, jd, L(stage_time(i)));
+            const L scale = L(h * Cfg::A[i][i]);
 #pragma unroll
             for (int k = 0; k < n; ++k) out[i * n + k] = jacobi_divide(v[i * n + k], scale, jd[k]);
         }
@@ -90,22 +101,22 @@ struct FirkProblem {
         for (int i = 0; i < s; ++i) {
             real Y[n], f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1];
             combine<true>(i, u, Y);
-            Sys::dxdt(Y, params, stage_drivers(i), obs, f, stage_time(i));
+            Sys::dxdt(Y, params, drivers + i * nd, obs, f, stage_time(i));
 #pragma unroll
             for (int k = 0; k < n; ++k) out[i * n + k] = u[i * n + k] - h * f[k];
         }
     }
-    __device__ __forceinline__ real krylov_norm2(const real* vals) const {
-        real nrm2 = real(0);
+    __device__ __forceinline__ L krylov_norm2(const L* vals) const {
+        L nrm2 = L(0);
 #pragma unroll
         for (int e = 0; e < W; ++e) {
             const int k = e % n;
-            real tol = Cfg::krylov_atol[k] + Cfg::krylov_rtol[k] * dabs(base[k]);
-            tol = pmax(tol, real(1e-16));
-            const real r = dabs(vals[e]) / tol;
+            L tol = L(Cfg::krylov_atol[k]) + L(Cfg::krylov_rtol[k]) * dabs(L(base[k]));
+            tol = pmax(tol, L(1e-16));
+            const L r = dabs(vals[e]) / tol;
             nrm2 += r * r;
         }
-        return nrm2 * real(1.0 / double(W));
+        return nrm2 * L(1.0 / double(W));
     }
     __device__ __forceinline__ real correction_norm2(const real* vals) const {
         real nrm2 = real(0);
@@ -202,10 +213,14 @@ struct FIRKStep {
         }
 
         FirkProblem<Cfg, Sys> P;
+        LinInputs<Cfg, (nd > 0 ? s * nd : 1)> li;
+        li.bind(ctx.params.data(), stack.data());
         P.u = stage_increment.data();
         P.base = state.data();
         P.params = ctx.params.data();
         P.drivers = stack.data();
+        P.params_l = li.params;
+        P.drivers_l = li.drivers;
         P.t = current_time;
         P.h = dt;
         status |= newton_solve<Cfg>(P, theta, counters);

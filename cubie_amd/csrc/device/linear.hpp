@@ -1,6 +1,6 @@
 // Matrix-free implicit-stage machinery: weighted norms, the linearised stage
-// operator, the truncated Neumann preconditioner, minimal-residual /
-// steepest-descent and BiCGSTAB linear solvers, and the Newton iteration.
+// operator, the truncated Neumann and Jacobi preconditioners, minimal-residual
+// / steepest-descent and BiCGSTAB linear solvers, and the Newton iteration.
 //
 // Behaviour follows the reference:
 //   operator   beta*v - gamma*a*h*J(base + a*u) v
@@ -22,11 +22,17 @@
 // beta = gamma = 1 (the reference's ImplicitStepConfig defaults) and the mass
 // matrix is the identity.
 //
-// A stage "problem" type supplies the width W and the device callbacks
-// (operator, preconditioner, residual, norms) over register-resident
-// vectors; the solvers below are written once against that interface and
-// serve DIRK stages (W = n), coupled FIRK stages (W = s*n) and Rosenbrock
-// stages (linear solves only).
+// A stage "problem" type supplies the width W and the device callbacks over
+// register-resident vectors: residual() and correction_norm2() in the
+// working precision `real`, and op() / neumann() / jacobi() /
+// krylov_norm2() in the linear-solve precision L = Cfg::lin_real. The
+// solvers below are written once against that interface and serve DIRK
+// stages (W = n), coupled FIRK stages (W = s*n) and Rosenbrock stages
+// (linear solves only). L equals `real` except in the mixed-precision mode,
+// where the Newton iterate, its residual and its norm stay in FP64 and the
+// Krylov iterations (the Jacobian-vector products and vector updates, most of
+// the flops) run in FP32: an inexact Newton method whose outer residual is
+// exact to FP64.
 #pragma once
 
 namespace cubie {
@@ -34,37 +40,12 @@ namespace cubie {
 enum { LIN_MR = 0, LIN_SD = 1, LIN_BICGSTAB = 2 };
 enum { PRE_NEUMANN = 0, PRE_JACOBI = 1 };
 
-// Preconditioner application: the configured kinds in order (a single kind,
-// or a chain v -> P0 -> P1 -> ... as codegen/preconditioners.py:1113-1232
-// composes them), each supplied by the problem type as neumann() / jacobi().
-template <typename Cfg, typename Prob>
-__device__ __forceinline__ void precondition(const Prob& P, const real* v, real* out) {
-    constexpr int K = Cfg::precond_count;
-    if constexpr (K == 1) {
-        if constexpr (Cfg::precond_kinds[0] == PRE_JACOBI) P.jacobi(v, out);
-        else P.neumann(v, out);
-    } else {
-        real buf[2][Prob::W];
-        const real* src = v;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            real* dst = (k == K - 1) ? out : buf[k & 1];
-            if (Cfg::precond_kinds[k] == PRE_JACOBI) P.jacobi(src, dst);
-            else P.neumann(src, dst);
-            src = dst;
-        }
-    }
-}
+template <typename A, typename B>
+struct same_type { static constexpr bool value = false; };
+template <typename A>
+struct same_type<A, A> { static constexpr bool value = true; };
 
-// One Jacobi entry: v / d with d = 1 - scale * J_ii, |d| floored at 1e-16
-// (codegen/preconditioners.py:554-594).
-__device__ __forceinline__ real jacobi_divide(real v, real scale, real jii) {
-    const real d = real(1) - scale * jii;
-    const real safe = dabs(d) >= real(1e-16) ? d : real(1e-16);
-    return v / safe;
-}
-
-template <typename real>
+template <typename T>
 struct SolverConsts;
 template <>
 struct SolverConsts<double> {
@@ -81,42 +62,73 @@ struct SolverConsts<float> {
     static constexpr float stagnation_eps = 0.034526698300124390f;      // 100*sqrt(eps)
 };
 
-template <typename real>
-__device__ __forceinline__ real clampc(real x) {
-    constexpr real c = SolverConsts<real>::clamp;
+template <typename T>
+__device__ __forceinline__ T clampc(T x) {
+    constexpr T c = SolverConsts<T>::clamp;
     x = x > c ? c : x;
     x = x < -c ? -c : x;
     return x;
 }
-template <typename real>
-__device__ __forceinline__ real clamp_pos(real x) {
-    constexpr real c = SolverConsts<real>::clamp;
+template <typename T>
+__device__ __forceinline__ T clamp_pos(T x) {
+    constexpr T c = SolverConsts<T>::clamp;
     return x > c ? c : x;
+}
+
+// Preconditioner application: the configured kinds in order (a single kind,
+// or a chain v -> P0 -> P1 -> ... as codegen/preconditioners.py:1113-1232
+// composes them), each supplied by the problem type as neumann() / jacobi().
+template <typename Cfg, typename L, typename Prob>
+__device__ __forceinline__ void precondition(const Prob& P, const L* v, L* out) {
+    constexpr int K = Cfg::precond_count;
+    if constexpr (K == 1) {
+        if constexpr (Cfg::precond_kinds[0] == PRE_JACOBI) P.jacobi(v, out);
+        else P.neumann(v, out);
+    } else {
+        L buf[2][Prob::W];
+        const L* src = v;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            L* dst = (k == K - 1) ? out : buf[k & 1];
+            if (Cfg::precond_kinds[k] == PRE_JACOBI) P.jacobi(src, dst);
+            else P.neumann(src, dst);
+            src = dst;
+        }
+    }
+}
+
+// One Jacobi entry: v / d with d = 1 - scale * J_ii, |d| floored at 1e-16
+// (codegen/preconditioners.py:554-594).
+template <typename T>
+__device__ __forceinline__ T jacobi_divide(T v, T scale, T jii) {
+    const T d = T(1) - scale * jii;
+    const T safe = dabs(d) >= T(1e-16) ? d : T(1e-16);
+    return v / safe;
 }
 
 // Preconditioned minimal-residual / steepest-descent solve of A x = rhs.
 // rhs is overwritten by the running residual; x holds the initial guess.
-template <typename Cfg, typename Prob>
-__device__ __forceinline__ int mr_solve(Prob& P, real* rhs, real* x, bool x_is_zero, int& iters) {
+template <typename Cfg, typename L, typename Prob>
+__device__ __forceinline__ int mr_solve(Prob& P, L* rhs, L* x, bool x_is_zero, int& iters) {
     constexpr int W = Prob::W;
-    const real rhs_norm2 = P.krylov_norm2(rhs);
-    const real tol = Cfg::krylov_floor + Cfg::krylov_reduction * real(dsqrt(rhs_norm2));
-    const real tol2 = tol * tol;
-    real temp[W], z[W];
+    const L rhs_norm2 = P.krylov_norm2(rhs);
+    const L tol = L(Cfg::krylov_floor) + L(Cfg::krylov_reduction) * L(dsqrt(rhs_norm2));
+    const L tol2 = tol * tol;
+    L temp[W], z[W];
     if (!x_is_zero) {
         P.op(x, temp);
 #pragma unroll
         for (int i = 0; i < W; ++i) rhs[i] = rhs[i] - temp[i];
     }
-    real acc = P.krylov_norm2(rhs);
+    L acc = P.krylov_norm2(rhs);
     bool converged = acc <= tol2;
     int it = 0;
     for (int k = 0; k < Cfg::krylov_max_iters; ++k) {
         if (converged) break;
         ++it;
-        precondition<Cfg>(P, rhs, z);
+        precondition<Cfg, L>(P, rhs, z);
         P.op(z, temp);
-        real num = real(0), den = real(0);
+        L num = L(0), den = L(0);
 #pragma unroll
         for (int i = 0; i < W; ++i) {
             if constexpr (Cfg::linear_solver == LIN_SD) {
@@ -127,7 +139,7 @@ __device__ __forceinline__ int mr_solve(Prob& P, real* rhs, real* x, bool x_is_z
                 den += temp[i] * temp[i];
             }
         }
-        const real alpha = den != real(0) ? num / den : real(0);
+        const L alpha = den != L(0) ? num / den : L(0);
 #pragma unroll
         for (int i = 0; i < W; ++i) {
             x[i] += alpha * z[i];
@@ -141,32 +153,32 @@ __device__ __forceinline__ int mr_solve(Prob& P, real* rhs, real* x, bool x_is_z
 }
 
 // Preconditioned BiCGSTAB with the reference's clamps and breakdown guards.
-template <typename Cfg, typename Prob>
-__device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool x_is_zero, int& iters) {
+template <typename Cfg, typename L, typename Prob>
+__device__ __forceinline__ int bicgstab_solve(Prob& P, L* rhs, L* x, bool x_is_zero, int& iters) {
     constexpr int W = Prob::W;
-    constexpr real brk = SolverConsts<real>::breakdown;
-    constexpr real dclamp = SolverConsts<real>::clamp;
-    const real rhs_norm2 = P.krylov_norm2(rhs);
-    const real tol = Cfg::krylov_floor + Cfg::krylov_reduction * real(dsqrt(rhs_norm2));
-    const real tol2 = tol * tol;
-    real r0_hat[W], p[W], v[W], tmp[W], s_hat[W];
+    constexpr L brk = SolverConsts<L>::breakdown;
+    constexpr L dclamp = SolverConsts<L>::clamp;
+    const L rhs_norm2 = P.krylov_norm2(rhs);
+    const L tol = L(Cfg::krylov_floor) + L(Cfg::krylov_reduction) * L(dsqrt(rhs_norm2));
+    const L tol2 = tol * tol;
+    L r0_hat[W], p[W], v[W], tmp[W], s_hat[W];
     if (!x_is_zero) {
         P.op(x, tmp);
     } else {
 #pragma unroll
-        for (int i = 0; i < W; ++i) tmp[i] = real(0);
+        for (int i = 0; i < W; ++i) tmp[i] = L(0);
     }
-    real rho_prev = real(0);
+    L rho_prev = L(0);
 #pragma unroll
     for (int i = 0; i < W; ++i) {
-        const real ax = clampc(tmp[i]);
-        const real ri = rhs[i] - ax;
+        const L ax = clampc(tmp[i]);
+        const L ri = rhs[i] - ax;
         rhs[i] = ri;
         r0_hat[i] = ri;
         p[i] = clampc(ri);
         rho_prev += clamp_pos(ri * ri);
     }
-    real acc = P.krylov_norm2(rhs);
+    L acc = P.krylov_norm2(rhs);
     bool converged = acc <= tol2;
     bool broken = false;
     bool finished = converged;
@@ -174,11 +186,11 @@ __device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool 
     for (int k = 0; k < Cfg::krylov_max_iters; ++k) {
         if (finished) break;
         ++it;
-        precondition<Cfg>(P, p, tmp);
+        precondition<Cfg, L>(P, p, tmp);
 #pragma unroll
         for (int i = 0; i < W; ++i) tmp[i] = clampc(tmp[i]);
         P.op(tmp, v);
-        real dot_r0v = real(0);
+        L dot_r0v = L(0);
 #pragma unroll
         for (int i = 0; i < W; ++i) {
             v[i] = clampc(v[i]);
@@ -187,8 +199,8 @@ __device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool 
         const bool alpha_overflow = dabs(rho_prev) > dabs(dot_r0v) * dclamp;
         broken = broken || (!finished && alpha_overflow);
         finished = converged || broken;
-        const real alpha = (dot_r0v != real(0) && !alpha_overflow) ? rho_prev / dot_r0v : real(0);
-        const real alpha_eff = finished ? real(0) : alpha;
+        const L alpha = (dot_r0v != L(0) && !alpha_overflow) ? rho_prev / dot_r0v : L(0);
+        const L alpha_eff = finished ? L(0) : alpha;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
             x[i] = x[i] + alpha_eff * tmp[i];
@@ -197,21 +209,21 @@ __device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool 
         acc = P.krylov_norm2(rhs);
         converged = converged || (acc <= tol2);
         finished = converged || broken;
-        precondition<Cfg>(P, rhs, s_hat);
+        precondition<Cfg, L>(P, rhs, s_hat);
 #pragma unroll
         for (int i = 0; i < W; ++i) s_hat[i] = clampc(s_hat[i]);
         P.op(s_hat, tmp);
-        real dot_ts = real(0), dot_tt = real(0);
+        L dot_ts = L(0), dot_tt = L(0);
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            const real ti = clampc(tmp[i]);
+            const L ti = clampc(tmp[i]);
             tmp[i] = ti;
             dot_ts += clampc(ti * rhs[i]);
             dot_tt += clamp_pos(ti * ti);
         }
         const bool omega_overflow = dabs(dot_ts) > dot_tt * dclamp;
-        const real omega = (dot_tt != real(0) && !omega_overflow) ? dot_ts / dot_tt : real(0);
-        const real omega_eff = finished ? real(0) : omega;
+        const L omega = (dot_tt != L(0) && !omega_overflow) ? dot_ts / dot_tt : L(0);
+        const L omega_eff = finished ? L(0) : omega;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
             x[i] = x[i] + omega_eff * s_hat[i];
@@ -219,7 +231,7 @@ __device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool 
         }
         acc = P.krylov_norm2(rhs);
         converged = converged || (acc <= tol2);
-        real rho_new = real(0);
+        L rho_new = L(0);
 #pragma unroll
         for (int i = 0; i < W; ++i) rho_new += clampc(r0_hat[i] * rhs[i]);
         const bool rho_bad = dabs(rho_new) < brk;
@@ -230,10 +242,10 @@ __device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool 
                                    (dabs(alpha) > dabs(omega) * dclamp);
         broken = broken || (!finished && beta_overflow);
         finished = converged || broken;
-        const real beta = !finished ? (rho_new / rho_prev) * (alpha / omega) : real(0);
+        const L beta = !finished ? (rho_new / rho_prev) * (alpha / omega) : L(0);
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            real pi = !finished ? rhs[i] + beta * (p[i] - omega * v[i]) : p[i];
+            L pi = !finished ? rhs[i] + beta * (p[i] - omega * v[i]) : p[i];
             p[i] = clampc(pi);
         }
         rho_prev = !finished ? rho_new : rho_prev;
@@ -242,16 +254,19 @@ __device__ __forceinline__ int bicgstab_solve(Prob& P, real* rhs, real* x, bool 
     return converged ? ST_SUCCESS : (broken ? ST_BICGSTAB_BREAKDOWN : ST_LINEAR_MAX_ITERS);
 }
 
-template <typename Cfg, typename Prob>
-__device__ __forceinline__ int linear_solve(Prob& P, real* rhs, real* x, bool x_is_zero, int& iters) {
-    if constexpr (Cfg::linear_solver == LIN_BICGSTAB) return bicgstab_solve<Cfg>(P, rhs, x, x_is_zero, iters);
-    else return mr_solve<Cfg>(P, rhs, x, x_is_zero, iters);
+template <typename Cfg, typename L, typename Prob>
+__device__ __forceinline__ int linear_solve(Prob& P, L* rhs, L* x, bool x_is_zero, int& iters) {
+    if constexpr (Cfg::linear_solver == LIN_BICGSTAB) return bicgstab_solve<Cfg, L>(P, rhs, x, x_is_zero, iters);
+    else return mr_solve<Cfg, L>(P, rhs, x, x_is_zero, iters);
 }
 
 // Newton iteration on the stage increment P.u (NLNewton acceptance rules).
-// theta_store carries the contraction estimate between solves.
+// theta_store carries the contraction estimate between solves. The residual,
+// the iterate and the correction norm are in `real`; each correction is
+// solved in Cfg::lin_real (converted at this boundary when they differ).
 template <typename Cfg, typename Prob>
 __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* counters) {
+    using L = typename Cfg::lin_real;
     constexpr int W = Prob::W;
     const real kappa = real(0.01);
     const real first_bound = real(1.0e-5);
@@ -269,13 +284,26 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         if (converged || failed) break;
         ++iteration;
         P.residual(residual);
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-            residual[i] = -residual[i];
-            delta[i] = real(0);
-        }
         int kit = 0;
-        const int lin = linear_solve<Cfg>(P, residual, delta, true, kit);
+        int lin;
+        if constexpr (same_type<L, real>::value) {
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                residual[i] = -residual[i];
+                delta[i] = real(0);
+            }
+            lin = linear_solve<Cfg, L>(P, residual, delta, true, kit);
+        } else {
+            L rl[W], dl[W];
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                rl[i] = L(-residual[i]);
+                dl[i] = L(0);
+            }
+            lin = linear_solve<Cfg, L>(P, rl, dl, true, kit);
+#pragma unroll
+            for (int i = 0; i < W; ++i) delta[i] = real(dl[i]);
+        }
         krylov_total += kit;
         last_lin = lin;
         ++iters;
@@ -308,9 +336,34 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
     return converged ? ST_SUCCESS : fail_bits;
 }
 
+// Parameters and one driver vector in the linear-solve precision: the
+// caller's arrays when the precisions agree, converted copies otherwise.
+template <typename Cfg, int NDrv = Cfg::n_drivers>
+struct LinInputs {
+    using L = typename Cfg::lin_real;
+    Vec<L, Cfg::n_params> pl;
+    Vec<L, NDrv> dl;
+    const L* params;
+    const L* drivers;
+    __device__ __forceinline__ void bind(const real* p, const real* d) {
+        if constexpr (same_type<L, real>::value) {
+            params = p;
+            drivers = d;
+        } else {
+#pragma unroll
+            for (int i = 0; i < Cfg::n_params; ++i) pl[i] = L(p[i]);
+#pragma unroll
+            for (int i = 0; i < NDrv; ++i) dl[i] = L(d[i]);
+            params = pl.data();
+            drivers = dl.data();
+        }
+    }
+};
+
 // Single-stage (DIRK) implicit problem: unknown u, stage state base + a*u.
 template <typename Cfg, typename Sys>
 struct DirkStageProblem {
+    using L = typename Cfg::lin_real;
     static constexpr int n = Cfg::n;
     static constexpr int W = n;
     real* u;               // stage increment (Newton iterate)
@@ -318,60 +371,61 @@ struct DirkStageProblem {
     const real* step_start;
     const real* params;
     const real* drivers;
+    const L* params_l;     // the same, in the linear-solve precision
+    const L* drivers_l;
     real t, h, a;
 
-    __device__ __forceinline__ void stage_state(real* Y) const {
+    __device__ __forceinline__ void stage_state_l(L* Y) const {
 #pragma unroll
-        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
+        for (int i = 0; i < n; ++i) Y[i] = L(base[i] + a * u[i]);
     }
-    __device__ __forceinline__ void jvp(const real* v, real* out) const {
-        real Y[n];
-        stage_state(Y);
-        Sys::jvp(Y, params, drivers, v, out, t);
-    }
-    __device__ __forceinline__ void op(const real* v, real* out) const {
-        real jv[n];
-        jvp(v, jv);
-        const real ah = a * h;
+    __device__ __forceinline__ void op(const L* v, L* out) const {
+        L Y[n], jv[n];
+        stage_state_l(Y);
+        Sys::jvp(Y, params_l, drivers_l, v, jv, L(t));
+        const L ah = L(a * h);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = v[i] - ah * jv[i];
     }
-    __device__ __forceinline__ void jacobi(const real* v, real* out) const {
-        real Y[n], jd[n];
-        stage_state(Y);
-        Sys::jac_diag(Y, params, drivers, jd, t);
-        const real scale = h * a;
+    __device__ __forceinline__ void jacobi(const L* v, L* out) const {
+        L Y[n], jd[n];
+        stage_state_l(Y);
+        Sys::jac_diag(Y, params_l, drivers_l, jd, L(t));
+        const L scale = L(h * a);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = jacobi_divide(v[i], scale, jd[i]);
     }
-    __device__ __forceinline__ void neumann(const real* v, real* out) const {
-        const real h_eff = h * a;
+    __device__ __forceinline__ void neumann(const L* v, L* out) const {
+        const L h_eff = L(h * a);
+        L Y[n];
+        stage_state_l(Y);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = v[i];
         for (int k = 0; k < Cfg::preconditioner_order; ++k) {
-            real jv[n];
-            jvp(out, jv);
+            L jv[n];
+            Sys::jvp(Y, params_l, drivers_l, out, jv, L(t));
 #pragma unroll
             for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * jv[i];
         }
     }
     __device__ __forceinline__ void residual(real* out) const {
         real Y[n], f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1];
-        stage_state(Y);
+#pragma unroll
+        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
         Sys::dxdt(Y, params, drivers, obs, f, t);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
     }
-    __device__ __forceinline__ real krylov_norm2(const real* vals) const {
-        real nrm2 = real(0);
+    __device__ __forceinline__ L krylov_norm2(const L* vals) const {
+        L nrm2 = L(0);
 #pragma unroll
         for (int i = 0; i < n; ++i) {
-            real tol = Cfg::krylov_atol[i] + Cfg::krylov_rtol[i] * dabs(base[i]);
-            tol = pmax(tol, real(1e-16));
-            const real r = dabs(vals[i]) / tol;
+            L tol = L(Cfg::krylov_atol[i]) + L(Cfg::krylov_rtol[i]) * dabs(L(base[i]));
+            tol = pmax(tol, L(1e-16));
+            const L r = dabs(vals[i]) / tol;
             nrm2 += r * r;
         }
-        return nrm2 * real(Cfg::inv_n);
+        return nrm2 * L(Cfg::inv_n);
     }
     __device__ __forceinline__ real correction_norm2(const real* vals) const {
         real nrm2 = real(0);

@@ -7,8 +7,8 @@
 // failed-step 1e16 error, the controller, the counters, the two-stall
 // stagnation rule, the status fold on an irrecoverable end, the predicated
 // commit of t / state / drivers / observables and finally the saves and
-// summary updates. Time is accumulated in double; everything else is in
-// Cfg::real (or, in compensated mode, real plus a running correction term).
+// summary updates. Time is accumulated in double in every mode; the state is
+// in Cfg::real, with a Kahan correction term in the compensated explicit mode.
 //
 // The loop is split into begin() / advance() / status so that two kernels can
 // share it:
@@ -45,6 +45,7 @@ struct Integrator {
     Controller<Cfg> ctrl;
     Summaries<Cfg> summ;
     Vec<real, n> state, proposal;
+    Vec<real, (Cfg::kahan_state ? n : 1)> state_c;   // Kahan compensation of state
     Vec<real, Cfg::n_error> error;
     long long run;
     double t;
@@ -59,7 +60,8 @@ struct Integrator {
     __device__ __forceinline__ void begin(const cubie_solve_args& a, long long r) {
         run = r;
         const long long N = a.n_runs;
-        ctx.drv.coeffs = static_cast<const real*>(a.driver_coeffs);
+        using io_real = typename Cfg::io_real;
+        ctx.drv.coeffs = static_cast<const io_real*>(a.driver_coeffs);
         stepper.init();
         ctrl.init();
         summ.zero();
@@ -69,7 +71,7 @@ struct Integrator {
         save_stop = real(a.save_stop);
         summary_stop = real(a.summary_stop);
         rtol_run = nreal(0);
-        if constexpr (Cfg::per_run_rtol) rtol_run = nreal(static_cast<const real*>(a.rtol_per_run)[run]);
+        if constexpr (Cfg::per_run_rtol) rtol_run = nreal(static_cast<const io_real*>(a.rtol_per_run)[run]);
         t = double(t0);
         t_prec = real(t);
         t_end = real(settling + t0 + duration);
@@ -86,18 +88,20 @@ struct Integrator {
         update_idx = 0;
         next_save = real(settling + t0);
         next_update = real(settling + t0);
-        const real* inits = static_cast<const real*>(a.inits);
-        const real* params = static_cast<const real*>(a.params);
+        const io_real* inits = static_cast<const io_real*>(a.inits);
+        const io_real* params = static_cast<const io_real*>(a.params);
 #pragma unroll
-        for (int i = 0; i < n; ++i) state[i] = inits[i * N + run];
+        for (int i = 0; i < n; ++i) state[i] = real(inits[i * N + run]);
 #pragma unroll
-        for (int i = 0; i < Cfg::n_params; ++i) ctx.params[i] = params[i * N + run];
+        for (int i = 0; i < Cfg::n_params; ++i) ctx.params[i] = real(params[i * N + run]);
 #pragma unroll
         for (int i = 0; i < Cfg::n_drivers; ++i) { ctx.drivers[i] = real(0); ctx.proposed_drivers[i] = real(0); }
 #pragma unroll
         for (int i = 0; i < Cfg::n_obs; ++i) { ctx.observables[i] = real(0); ctx.proposed_observables[i] = real(0); }
 #pragma unroll
         for (int i = 0; i < n; ++i) proposal[i] = real(0);
+#pragma unroll
+        for (int i = 0; i < (Cfg::kahan_state ? n : 1); ++i) state_c[i] = real(0);
         if constexpr (Cfg::n_drivers > 0) ctx.drv.eval(t_prec, ctx.drivers.data());
         if constexpr (Cfg::n_obs > 0)
             Sys::observables(state.data(), ctx.params.data(), ctx.drivers.data(),
@@ -186,8 +190,7 @@ struct Integrator {
             counters_since_save[3] += accept ? 0 : 1;
         }
 
-        double t_proposal = t + double(dt_eff);
-        if constexpr (Cfg::compensated) t_proposal = stepper.time_after(t, dt_eff, accept);
+        const double t_proposal = t + double(dt_eff);
         stagnant = (t_proposal == t) ? stagnant + 1 : 0;
         const bool is_stagnant = stagnant >= 2;
         if (is_stagnant) iteration_status |= ST_STAGNATION;
@@ -197,9 +200,21 @@ struct Integrator {
 
         t = accept ? t_proposal : t;
         t_prec = real(t);
-        if constexpr (Cfg::compensated) stepper.commit(accept);
+        if constexpr (Cfg::kahan_state) {
+            // y += increment with Kahan compensation: the bits the FP32 sum
+            // drops are carried into the next step's update.
 #pragma unroll
-        for (int i = 0; i < n; ++i) state[i] = accept ? proposal[i] : state[i];
+            for (int i = 0; i < n; ++i) {
+                const real yv = stepper.increment[i] + state_c[i];
+                const real sum = state[i] + yv;
+                const real comp = yv - (sum - state[i]);
+                state_c[i] = accept ? comp : state_c[i];
+                state[i] = accept ? sum : state[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < n; ++i) state[i] = accept ? proposal[i] : state[i];
+        }
 #pragma unroll
         for (int i = 0; i < Cfg::n_drivers; ++i)
             ctx.drivers[i] = accept ? ctx.proposed_drivers[i] : ctx.drivers[i];

@@ -253,8 +253,9 @@ struct Summaries {
         constexpr int OH = Cfg::summ_out_height;
         const long long N = a.n_runs;
         const bool store = row < a.summary_rows;
+        using io_real = typename Cfg::io_real;
         if constexpr (Cfg::summarise_state) {
-            real* out = static_cast<real*>(a.state_summ_out);
+            io_real* out = static_cast<io_real*>(a.state_summ_out);
 #pragma unroll
             for (int v = 0; v < NS; ++v) {
                 real tmp[OH > 0 ? OH : 1];
@@ -262,12 +263,12 @@ struct Summaries {
                 if (store) {
 #pragma unroll
                     for (int k = 0; k < OH; ++k)
-                        out[(row * (NS * OH) + v * OH + k) * N + run] = tmp[k];
+                        out[(row * (NS * OH) + v * OH + k) * N + run] = io_real(tmp[k]);
                 }
             }
         }
         if constexpr (Cfg::summarise_obs) {
-            real* out = static_cast<real*>(a.obs_summ_out);
+            io_real* out = static_cast<io_real*>(a.obs_summ_out);
 #pragma unroll
             for (int v = 0; v < NO; ++v) {
                 real tmp[OH > 0 ? OH : 1];
@@ -275,7 +276,7 @@ struct Summaries {
                 if (store) {
 #pragma unroll
                     for (int k = 0; k < OH; ++k)
-                        out[(row * (NO * OH) + v * OH + k) * N + run] = tmp[k];
+                        out[(row * (NO * OH) + v * OH + k) * N + run] = io_real(tmp[k]);
                 }
             }
         }
@@ -287,25 +288,25 @@ template <typename Cfg, typename S, typename O>
 __device__ __forceinline__ void save_state(const cubie_solve_args& a, long long run, long long row,
                                            const S& state, const O& obs, const int* counters,
                                            typename Cfg::real t_prec) {
-    using real = typename Cfg::real;
+    using io_real = typename Cfg::io_real;
     if (row >= a.out_rows) return;
     const long long N = a.n_runs;
     constexpr int W = Cfg::n_saved_states + (Cfg::save_time ? 1 : 0);
     if constexpr (Cfg::save_state) {
-        real* out = static_cast<real*>(a.state_out);
+        io_real* out = static_cast<io_real*>(a.state_out);
 #pragma unroll
         for (int k = 0; k < Cfg::n_saved_states; ++k)
-            out[(row * W + k) * N + run] = state[Cfg::saved_state_idx[k]];
+            out[(row * W + k) * N + run] = io_real(state[Cfg::saved_state_idx[k]]);
     }
     if constexpr (Cfg::save_time) {
-        real* out = static_cast<real*>(a.state_out);
-        out[(row * W + Cfg::n_saved_states) * N + run] = t_prec;
+        io_real* out = static_cast<io_real*>(a.state_out);
+        out[(row * W + Cfg::n_saved_states) * N + run] = io_real(t_prec);
     }
     if constexpr (Cfg::save_obs) {
-        real* out = static_cast<real*>(a.observables_out);
+        io_real* out = static_cast<io_real*>(a.observables_out);
 #pragma unroll
         for (int k = 0; k < Cfg::n_saved_obs; ++k)
-            out[(row * Cfg::n_saved_obs + k) * N + run] = obs[Cfg::saved_obs_idx[k]];
+            out[(row * Cfg::n_saved_obs + k) * N + run] = io_real(obs[Cfg::saved_obs_idx[k]]);
     }
     if constexpr (Cfg::save_counters) {
 #pragma unroll

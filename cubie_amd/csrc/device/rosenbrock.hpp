@@ -12,70 +12,72 @@
 // 256-335 with a_ij = 1 and beta = 1; the Neumann series uses h_eff = h gamma
 // and the Jacobi preconditioner the diagonal of I - gamma h J. Small systems
 // keep the dense Jacobian in registers; larger ones re-evaluate J v at the
-// step-start state. Like the reference, the step reports no Newton / Krylov
-// counts to the loop.
+// step-start state. The linear solves run in Cfg::lin_real (FP32 in the
+// mixed-precision mode, with right-hand sides and stage sums in FP64). Like
+// the reference, the step reports no Newton / Krylov counts to the loop.
 #pragma once
 
 namespace cubie {
 
 template <typename Cfg, typename Sys>
 struct RosenbrockProblem {
+    using L = typename Cfg::lin_real;
     static constexpr int n = Cfg::n;
     static constexpr int W = n;
     static constexpr bool cache_jac = Cfg::n <= 8;
-    const real* state;
-    const real* params;
-    const real* drivers;
-    const real* J;    // dense row-major Jacobian when cache_jac
-    const real* Jd;   // Jacobian diagonal
+    const L* state;     // y_n, in the linear-solve precision
+    const L* params;
+    const L* drivers;
+    const L* J;         // dense row-major Jacobian when cache_jac
+    const L* Jd;        // Jacobian diagonal
     real t, h;
 
-    __device__ __forceinline__ void jv(const real* v, real* out) const {
+    __device__ __forceinline__ void jv(const L* v, L* out) const {
         if constexpr (cache_jac) {
 #pragma unroll
             for (int i = 0; i < n; ++i) {
-                real acc = real(0);
+                L acc = L(0);
 #pragma unroll
                 for (int j = 0; j < n; ++j) acc += J[i * n + j] * v[j];
                 out[i] = acc;
             }
         } else {
-            Sys::jvp(state, params, drivers, v, out, t);
+            Sys::jvp(state, params, drivers, v, out, L(t));
         }
     }
-    __device__ __forceinline__ void op(const real* v, real* out) const {
-        real tmp[n];
+    __device__ __forceinline__ void op(const L* v, L* out) const {
+        L tmp[n];
         jv(v, tmp);
-        const real gh = Cfg::gamma * h;
+        const L gh = L(Cfg::gamma * h);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = v[i] - gh * tmp[i];
     }
-    __device__ __forceinline__ void neumann(const real* v, real* out) const {
-        const real h_eff = h * Cfg::gamma;
+    __device__ __forceinline__ void neumann(const L* v, L* out) const {
+        const L h_eff = L(h * Cfg::gamma);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = v[i];
         for (int k = 0; k < Cfg::preconditioner_order; ++k) {
-            real tmp[n];
+            L tmp[n];
             jv(out, tmp);
 #pragma unroll
             for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * tmp[i];
         }
     }
-    __device__ __forceinline__ void jacobi(const real* v, real* out) const {
-        const real scale = Cfg::gamma * h;
+    __device__ __forceinline__ void jacobi(const L* v, L* out) const {
+        const L scale = L(Cfg::gamma * h);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = jacobi_divide(v[i], scale, Jd[i]);
     }
-    __device__ __forceinline__ real krylov_norm2(const real* vals) const {
-        real nrm2 = real(0);
+    __device__ __forceinline__ L krylov_norm2(const L* vals) const {
+        L nrm2 = L(0);
 #pragma unroll
         for (int i = 0; i < n; ++i) {
-            real tol = Cfg::krylov_atol[i] + Cfg::krylov_rtol[i] * dabs(state[i]);
-            tol = pmax(tol, real(1e-16));
-            const real r = dabs(vals[i]) / tol;
+            L tol = L(Cfg::krylov_atol[i]) + L(Cfg::krylov_rtol[i]) * dabs(state[i]);
+            tol = pmax(tol, L(1e-16));
+            const L r = dabs(vals[i]) / tol;
             nrm2 += r * r;
         }
-        return nrm2 * real(Cfg::inv_n);
+        return nrm2 * L(Cfg::inv_n);
     }
 };
 
@@ -92,6 +94,22 @@ struct RosenbrockStep {
         for (int i = 0; i < n; ++i) stage_increment[i] = real(0);
     }
 
+    // (I - gamma h J) x = rhs from the guess in x, in the linear-solve precision.
+    __device__ __forceinline__ int solve(Prob& P, real* rhs, real* x, int& it) {
+        using L = typename Cfg::lin_real;
+        if constexpr (same_type<L, real>::value) {
+            return linear_solve<Cfg, L>(P, rhs, x, false, it);
+        } else {
+            L rl[n], xl[n];
+#pragma unroll
+            for (int i = 0; i < n; ++i) { rl[i] = L(rhs[i]); xl[i] = L(x[i]); }
+            const int st = linear_solve<Cfg, L>(P, rl, xl, false, it);
+#pragma unroll
+            for (int i = 0; i < n; ++i) x[i] = real(xl[i]);
+            return st;
+        }
+    }
+
     template <typename Ctx>
     __device__ __forceinline__ int step(Ctx& ctx, const Vec<real, n>& state, Vec<real, n>& proposed,
                                         Vec<real, Cfg::n_error>& error, real dt, real time,
@@ -102,15 +120,22 @@ struct RosenbrockStep {
         const real inv_dt = real(1) / dt;
         const real* params = ctx.params.data();
 
-        // Jacobian at the step start
-        Vec<real, (Prob::cache_jac ? n * n : 1)> J;
-        Vec<real, n> Jd;
+        // Jacobian at the step start, in the linear-solve precision
+        using L = typename Cfg::lin_real;
+        LinInputs<Cfg> li;
+        li.bind(params, ctx.drivers.data());
+        Vec<L, n> state_l;
+#pragma unroll
+        for (int i = 0; i < n; ++i) state_l[i] = L(state[i]);
+        const L t_l = L(current_time);
+        Vec<L, (Prob::cache_jac ? n * n : 1)> J;
+        Vec<L, n> Jd;
         if constexpr (Prob::cache_jac) {
-            Sys::jac(state.data(), params, ctx.drivers.data(), J.data(), current_time);
+            Sys::jac(state_l.data(), li.params, li.drivers, J.data(), t_l);
 #pragma unroll
             for (int i = 0; i < n; ++i) Jd[i] = J[i * n + i];
         } else {
-            Sys::jac_diag(state.data(), params, ctx.drivers.data(), Jd.data(), current_time);
+            Sys::jac_diag(state_l.data(), li.params, li.drivers, Jd.data(), t_l);
         }
 
         // dt * total time derivative at (t_n, y_n)
@@ -131,9 +156,9 @@ struct RosenbrockStep {
         }
 
         Prob P;
-        P.state = state.data();
-        P.params = params;
-        P.drivers = ctx.drivers.data();
+        P.state = state_l.data();
+        P.params = li.params;
+        P.drivers = li.drivers;
         P.J = J.data();
         P.Jd = Jd.data();
         P.t = current_time;
@@ -152,7 +177,7 @@ struct RosenbrockStep {
             rhs[i] = v * Cfg::gamma;
         }
         int it = 0;
-        status |= linear_solve<Cfg>(P, rhs, stage_increment.data(), false, it);
+        status |= solve(P, rhs, stage_increment.data(), it);
 #pragma unroll
         for (int i = 0; i < n; ++i) {
             store[i] = stage_increment[i];
@@ -198,7 +223,7 @@ struct RosenbrockStep {
             }
 #pragma unroll
             for (int i = 0; i < n; ++i) stage_increment[i] = store[(si - 1) * n + i];
-            status |= linear_solve<Cfg>(P, rhs, stage_increment.data(), false, it);
+            status |= solve(P, rhs, stage_increment.data(), it);
 #pragma unroll
             for (int i = 0; i < n; ++i) {
                 store[si * n + i] = stage_increment[i];

@@ -104,14 +104,26 @@ class KernelSpec:
     block_size: int = 256
 
     # -- helpers ---------------------------------------------------------------
+    @property
+    def mixed_implicit(self) -> bool:
+        """Compensated mode of an implicit family: FP64 outer iteration,
+        FP32 Krylov solves."""
+        return self.compensated and self.algorithm.is_implicit
+
+    @property
+    def work_precision(self):
+        """Precision of the loop and stepper arithmetic (``Cfg::real``)."""
+        return np.float64 if self.mixed_implicit else self.precision
+
     def _lit(self, v) -> str:
-        return c_literal(v, self.precision)
+        return c_literal(v, self.work_precision)
 
     def _real_name(self) -> str:
-        return "double" if self.precision is np.float64 else "float"
+        return "double" if self.work_precision is np.float64 else "float"
 
     def config_struct(self) -> str:
-        p = self.precision
+        p = self.precision           # precision-dependent choices (limits, eps)
+        wp = self.work_precision     # type of the real-valued constants
         s = self.system
         alg = self.algorithm
         tab = alg.tableau
@@ -125,8 +137,14 @@ class KernelSpec:
         a = L.append
         a("struct Cfg {")
         a(f"  using real = {self._real_name()};")
+        io_real = "double" if p is np.float64 else "float"
+        a(f"  using io_real = {io_real};           // global arrays (system precision)")
+        lin_real = "float" if self.mixed_implicit else self._real_name()
+        a(f"  using lin_real = {lin_real};          // Krylov solves")
         norm_real = "double" if (self.compensated or p is np.float64) else "float"
         a(f"  using norm_real = {norm_real};")
+        kahan = self.compensated and not self.algorithm.is_implicit
+        a(f"  static constexpr bool kahan_state = {str(kahan).lower()};")
         a(f"  static constexpr int n = {n};")
         a(f"  static constexpr int n_params = {s.num_parameters};")
         a(f"  static constexpr int n_obs = {s.num_observables};")
@@ -140,8 +158,8 @@ class KernelSpec:
         a(f"  static constexpr bool deterministic_pow = {str(self.deterministic_pow).lower()};")
         # controller
         a(f"  static constexpr int controller = {ctrl.id};")
-        a(f"  static constexpr real atol[{max(n, 1)}] = {c_array(ctrl.atol, p)};")
-        a(f"  static constexpr real rtol[{max(n, 1)}] = {c_array(ctrl.rtol, p)};")
+        a(f"  static constexpr real atol[{max(n, 1)}] = {c_array(ctrl.atol, wp)};")
+        a(f"  static constexpr real rtol[{max(n, 1)}] = {c_array(ctrl.rtol, wp)};")
         for key in ("inv_n", "kp_exp", "ki_exp", "kd_exp", "order_exponent", "safety",
                     "min_gain", "max_gain", "dt_min", "dt_max", "deadband_min",
                     "deadband_max", "gain_numerator"):
@@ -215,19 +233,19 @@ class KernelSpec:
         a(f"  static constexpr int b_hat_row = {-1 if bhr is None else bhr};")
         a(f"  static constexpr bool accumulates_output = {str(br is None).lower()};")
         a(f"  static constexpr bool accumulates_error = {str(bhr is None).lower()};")
-        rows = ", ".join(c_array(r, p) for r in tab.a)
+        rows = ", ".join(c_array(r, wp) for r in tab.a)
         a(f"  static constexpr real A[{st}][{st}] = {{{rows}}};")
-        a(f"  static constexpr real b[{st}] = {c_array(tab.b, p)};")
-        a(f"  static constexpr real c[{st}] = {c_array(tab.c, p)};")
+        a(f"  static constexpr real b[{st}] = {c_array(tab.b, wp)};")
+        a(f"  static constexpr real c[{st}] = {c_array(tab.c, wp)};")
         dvec = tab.d if alg.is_adaptive else [0.0] * st
-        a(f"  static constexpr real d[{st}] = {c_array(dvec, p)};")
+        a(f"  static constexpr real d[{st}] = {c_array(dvec, wp)};")
         bh = tab.b_hat if tab.b_hat is not None else [0.0] * st
-        a(f"  static constexpr real b_hat[{st}] = {c_array(bh, p)};")
+        a(f"  static constexpr real b_hat[{st}] = {c_array(bh, wp)};")
         if alg.family == "rosenbrock":
-            crow = ", ".join(c_array(r, p) for r in tab.C)
+            crow = ", ".join(c_array(r, wp) for r in tab.C)
             a(f"  static constexpr real rbC[{st}][{st}] = {{{crow}}};")
             a(f"  static constexpr real gamma = {self._lit(tab.gamma)};")
-            a(f"  static constexpr real gamma_stages[{st}] = {c_array(tab.gamma_stages, p)};")
+            a(f"  static constexpr real gamma_stages[{st}] = {c_array(tab.gamma_stages, wp)};")
         a(f"  static constexpr bool be_predictor = {str(alg.family == 'backwards_euler_pc').lower()};")
         # implicit solver settings
         im = self.implicit
@@ -240,7 +258,7 @@ class KernelSpec:
         a(f"  static constexpr int precond_count = {len(kinds)};")
         a(f"  static constexpr int precond_kinds[{len(kinds)}] = {c_int_array(kinds)};")
         for key in ("newton_atol", "newton_rtol", "krylov_atol", "krylov_rtol"):
-            a(f"  static constexpr real {key}[{max(n, 1)}] = {c_array(im.tol(key, n), p)};")
+            a(f"  static constexpr real {key}[{max(n, 1)}] = {c_array(im.tol(key, n), wp)};")
         # dense stage prediction (DIRK / FIRK)
         from .stage_predictors import dense_prediction_enabled, ratio_coefficients, source_stages
         dense = (alg.family in ("dirk", "firk")
@@ -251,7 +269,7 @@ class KernelSpec:
             first = 0 if not tab.first_stage_is_explicit else 1
             coeffs = coeffs[:, first:, :]
             powers, rows_ = coeffs.shape[0], coeffs.shape[1]
-            flat = ", ".join(c_array(coeffs[k].reshape(-1), p) for k in range(powers))
+            flat = ", ".join(c_array(coeffs[k].reshape(-1), wp) for k in range(powers))
             a(f"  static constexpr int pred_powers = {powers};")
             a(f"  static constexpr int pred_rows = {rows_};")
             a(f"  static constexpr int pred_first = {first};")
@@ -261,7 +279,7 @@ class KernelSpec:
             a("  static constexpr int pred_powers = 1;")
             a("  static constexpr int pred_rows = 1;")
             a("  static constexpr int pred_first = 0;")
-            a(f"  static constexpr real pred_coeffs[1][{st}] = {c_array([0.0] * st, p)};")
+            a(f"  static constexpr real pred_coeffs[1][{st}] = {c_array([0.0] * st, wp)};")
             a("  static constexpr real pred_max_ratio = real(0);")
         a(f"  static constexpr int pred_source[{st}] = {c_int_array(source_stages(tab.c))};")
         diag = [tab.a[i][i] for i in range(st)]

@@ -14,9 +14,12 @@ emitted as HIP C++ for hiprtc:
 * ``jac(y, p, d, J, t)``           dense J (row-major), for the adjoint
 * ``jacp(y, p, d, Jp, t)``         dense df/dparams (row-major)
 
-Every array argument is a pointer into a register-resident vector of the
-calling loop; after inlining the indices are constants. Constants of the
-system become exact literals. Common subexpressions are shared with
+Every function is a template on the scalar type ``real`` (the working
+precision of the caller: FP32 or FP64, and both in the mixed-precision mode,
+where residuals run in FP64 and Krylov products in FP32). Every array
+argument is a pointer into a register-resident vector of the calling loop;
+after inlining the indices are constants. Constants of the system become
+exact literals. Common subexpressions are shared with
 ``sympy.cse`` and counted by :func:`count_ops` for the roofline flop model.
 """
 
@@ -29,7 +32,7 @@ import numpy as np
 import sympy as sp
 from sympy.printing.c import C99CodePrinter
 
-from .._utils import c_literal
+
 from .symbolic import TIME_SYMBOL, SymbolicODE
 
 _FUNCS = {
@@ -49,7 +52,7 @@ class HipPrinter(C99CodePrinter):
         self.user_templates = dict(user_functions or {})
 
     def _lit(self, v: float) -> str:
-        return c_literal(v, self.precision)
+        return scalar_literal(v)
 
     def _print_Float(self, expr):
         return self._lit(float(expr))
@@ -150,6 +153,19 @@ class HipPrinter(C99CodePrinter):
         return _cname(str(expr))
 
 
+def scalar_literal(value: float) -> str:
+    """``real(<exact double>)``: the system functions are templates on the
+    scalar type, so a constant is written once as the exact double and
+    converted by the instantiation (float(double(v)) is the same value the
+    host gets from ``np.float32(v)``)."""
+    v = float(value)
+    if np.isnan(v):
+        return "real(__builtin_nan(\"\"))"
+    if np.isinf(v):
+        return "real(__builtin_huge_val())" if v > 0 else "real(-__builtin_huge_val())"
+    return f"real({float.hex(v)})"
+
+
 def _cname(name: str) -> str:
     return "v_" + re.sub(r"[^A-Za-z0-9_]", "_", name)
 
@@ -205,7 +221,7 @@ class SystemCodegen:
         for i, nm in enumerate(s.parameter_names):
             lines.append(f"    [[maybe_unused]] const real {_cname(nm)} = p[{i}];")
         for nm in s.constant_names:
-            lines.append(f"    [[maybe_unused]] constexpr real {_cname(nm)} = {c_literal(s.constants[nm], self.precision)};")
+            lines.append(f"    [[maybe_unused]] constexpr real {_cname(nm)} = {scalar_literal(s.constants[nm])};")
         if use_drivers:
             for i, nm in enumerate(s.driver_names):
                 lines.append(f"    [[maybe_unused]] const real {_cname(nm)} = d[{i}];")
@@ -247,7 +263,7 @@ class SystemCodegen:
         # dxdt (also writes observables so a stage leaves them current)
         targets = [(f"dy[{i}]", e) for i, e in enumerate(dx)]
         targets += [(f"obs[{i}]", e) for i, e in enumerate(obs)]
-        parts.append("  __device__ static __forceinline__ void dxdt(const real* __restrict__ y, "
+        parts.append("  template <typename real> __device__ static __forceinline__ void dxdt(const real* __restrict__ y, "
                      "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ obs, "
                      "real* __restrict__ dy, real t) {")
         parts += self._bindings()
@@ -255,7 +271,7 @@ class SystemCodegen:
         parts.append("    (void)y; (void)p; (void)d; (void)obs; (void)t;")
         parts.append("  }")
         # observables
-        parts.append("  __device__ static __forceinline__ void observables(const real* __restrict__ y, "
+        parts.append("  template <typename real> __device__ static __forceinline__ void observables(const real* __restrict__ y, "
                      "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ obs, real t) {")
         parts += self._bindings()
         parts += self._emit_block([(f"obs[{i}]", e) for i, e in enumerate(obs)], "observables")
@@ -266,7 +282,7 @@ class SystemCodegen:
         jv = []
         for fi in dx:
             jv.append(sp.Add(*[sp.diff(fi, yj) * vj for yj, vj in zip(s.state_symbols, vs)]))
-        parts.append("  __device__ static __forceinline__ void jvp(const real* __restrict__ y, "
+        parts.append("  template <typename real> __device__ static __forceinline__ void jvp(const real* __restrict__ y, "
                      "const real* __restrict__ p, const real* __restrict__ d, const real* __restrict__ v, "
                      "real* __restrict__ jv, real t) {")
         parts += self._bindings()
@@ -283,7 +299,7 @@ class SystemCodegen:
             for dsym, ddsym in zip(s.driver_symbols, dds):
                 e += sp.diff(fi, dsym) * ddsym
             dfdt.append(e)
-        parts.append("  __device__ static __forceinline__ void dfdt(const real* __restrict__ y, "
+        parts.append("  template <typename real> __device__ static __forceinline__ void dfdt(const real* __restrict__ y, "
                      "const real* __restrict__ p, const real* __restrict__ d, const real* __restrict__ dd, "
                      "real* __restrict__ out, real t) {")
         parts += self._bindings()
@@ -294,7 +310,7 @@ class SystemCodegen:
         parts.append("  }")
         # Jacobian diagonal for the Jacobi preconditioner
         Jd = [(f"Jd[{i}]", sp.diff(dx[i], s.state_symbols[i])) for i in range(n)]
-        parts.append("  __device__ static __forceinline__ void jac_diag(const real* __restrict__ y, "
+        parts.append("  template <typename real> __device__ static __forceinline__ void jac_diag(const real* __restrict__ y, "
                      "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ Jd, real t) {")
         parts += self._bindings()
         parts += self._emit_block(Jd, "jac_diag")
@@ -302,7 +318,7 @@ class SystemCodegen:
         parts.append("  }")
         # dense Jacobians for the adjoint
         J = [(f"J[{i * n + j}]", sp.diff(dx[i], s.state_symbols[j])) for i in range(n) for j in range(n)]
-        parts.append("  __device__ static __forceinline__ void jac(const real* __restrict__ y, "
+        parts.append("  template <typename real> __device__ static __forceinline__ void jac(const real* __restrict__ y, "
                      "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ J, real t) {")
         parts += self._bindings()
         parts += self._emit_block(J, "jac")
@@ -311,7 +327,7 @@ class SystemCodegen:
         npar = s.num_parameters
         Jp = [(f"Jp[{i * max(npar, 1) + j}]", sp.diff(dx[i], s.parameter_symbols[j]))
               for i in range(n) for j in range(npar)]
-        parts.append("  __device__ static __forceinline__ void jacp(const real* __restrict__ y, "
+        parts.append("  template <typename real> __device__ static __forceinline__ void jacp(const real* __restrict__ y, "
                      "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ Jp, real t) {")
         parts += self._bindings()
         parts += self._emit_block(Jp, "jacp")

@@ -28,6 +28,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 #include <algorithm>
 #include <limits>
@@ -153,6 +155,11 @@ struct OracleConfig {
     const int32_t* precond_kinds;
     // backward Euler: 0 warm start from the last increment, 1 explicit-Euler predictor
     int32_t be_predictor;
+    // optional per-run work tally (n_runs x 6): right-hand sides, Jacobian-vector
+    // products, (unused), time derivatives, Krylov iterations, Newton iterations
+    int64_t* work_out;
+    // mixed precision (implicit families): FP64 outer iteration, FP32 Krylov
+    int32_t mixed;
 };
 
 }  // extern "C"
@@ -219,18 +226,22 @@ struct Problem {
     SystemDesc sys;
     DriverEval<T> drv;
     int n;
+    mutable int64_t work[6] = {0, 0, 0, 0, 0, 0};
     void rhs(const T* y, const T* p, const T* d, T* obs, T* dy, T t) const {
+        ++work[0];
         sys_rhs<T>(sys, y, p, d, obs, dy, t);
     }
     void observables(const T* y, const T* p, const T* d, T* obs, T t) const {
         if (sys.n_obs > 0) sys_observables<T>(sys, y, p, d, obs, t);
     }
     void jvp(const T* y, const T* p, const T* d, const T* v, T* jv, T t) const {
+        ++work[1];
         sys_jvp<T>(sys, y, p, d, v, jv, t);
     }
     void drivers(T t, T* out) const { drv.eval(t, out); }
     void drivers_dt(T t, T* out) const { drv.eval(t, out, true); }
     void dfdt(const T* y, const T* p, const T* d, const T* ddt, T* out, T t) const {
+        ++work[3];
         sys_dfdt<T>(sys, y, p, d, ddt, out, t);
     }
 };
@@ -973,6 +984,9 @@ static int solve_typed(const OracleConfig* cfg, int64_t n_runs, const double* in
         Stepper st;
         st.init(&P);
         out.status[r] = run_loop<T, Stepper>(cfg, P, st, init, par, out, r);
+        if (cfg->work_out)
+            for (int k = 0; k < 6; ++k) cfg->work_out[r * 6 + k] = P.work[k];
+        for (int k = 0; k < 6; ++k) P.work[k] = 0;
     }
     return 0;
 }

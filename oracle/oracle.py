@@ -95,6 +95,8 @@ class OracleConfig(ctypes.Structure):
         ("dfdt_cb", DFDT_CB),
         ("precond_count", ctypes.c_int32), ("precond_kinds", _I),
         ("be_predictor", ctypes.c_int32),
+        ("work_out", ctypes.POINTER(ctypes.c_int64)),
+        ("mixed", ctypes.c_int32),
     ]
 
 
@@ -208,7 +210,14 @@ class OracleRun:
             keep.append(a)
             return a
 
-        cfg.precision_bits = 32 if prec is np.float32 else 64
+        # The mixed-precision (compensated) mode of an implicit family keeps the
+        # loop in FP64 and solves the Krylov systems in FP32; its inputs and
+        # outputs stay in the system precision.
+        self.mixed = (getattr(k, "precision_mode", "plain") == "compensated"
+                      and alg.is_implicit)
+        self.io_dtype = prec
+        cfg.precision_bits = 64 if (self.mixed or prec is np.float64) else 32
+        cfg.mixed = int(self.mixed)
         cfg.system_id = SYSTEM_IDS[system_id]
         cfg.n_states = sysm.num_states
         cfg.n_params = sysm.num_parameters
@@ -350,8 +359,8 @@ class OracleRun:
 
     def solve(self, inits: np.ndarray, params: np.ndarray, threads: int = 1,
               runs: Optional[range] = None) -> Dict[str, np.ndarray]:
-        inits = np.ascontiguousarray(inits, dtype=np.float64)
-        params = np.ascontiguousarray(params, dtype=np.float64)
+        inits = np.ascontiguousarray(np.asarray(inits, dtype=self.io_dtype), dtype=np.float64)
+        params = np.ascontiguousarray(np.asarray(params, dtype=self.io_dtype), dtype=np.float64)
         N = inits.shape[1]
         shapes = self.kernel.output_shapes(self.timing, self.dur_p, N)
         out = {}
@@ -359,6 +368,8 @@ class OracleRun:
             dt = np.int32 if key == "iteration_counters" else np.float64
             out[key] = np.zeros(max(int(np.prod(shp)), 1), dtype=dt)
         status = np.zeros(N, dtype=np.int32)
+        work_tally = np.zeros((N, 6), dtype=np.int64)
+        self.cfg.work_out = work_tally.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
         L = lib()
         pr = params if params.size else np.zeros((1, N))
         begin, end = (0, N) if runs is None else (runs.start, runs.stop)
@@ -381,7 +392,13 @@ class OracleRun:
                 raise RuntimeError(f"oracle_solve failed {rcs}")
         res = {k: (v[: int(np.prod(shapes[k]))].reshape(shapes[k]) if np.prod(shapes[k]) else
                    np.zeros(shapes[k], dtype=v.dtype)) for k, v in out.items()}
+        if self.mixed:
+            for key in ("state", "observables", "state_summaries", "observable_summaries"):
+                res[key] = res[key].astype(self.io_dtype).astype(np.float64)
         res["status_codes"] = status
+        # per-run tallies: rhs, jvp, -, dfdt, krylov iterations, newton iterations
+        res["work"] = work_tally
+        self.cfg.work_out = ctypes.POINTER(ctypes.c_int64)()
         return res
 
 

@@ -36,18 +36,47 @@ static void precondition(const OracleConfig* cfg, const Prob& Pb, const T* v, T*
     }
 }
 
-// Diagonal of J(Y) from unit-vector Jacobian-vector products.
-template <typename T>
-static void jacobian_diagonal(const Problem<T>* P, const T* Y, const T* params, const T* drivers,
-                              T t, T* diag) {
+// Diagonal of J(Y) from unit-vector Jacobian-vector products, below.
+// J(Y) v in precision V (the mixed mode evaluates the Krylov products in FP32).
+template <typename T, typename V>
+static void jvp_in(const Problem<T>* P, const V* Y, const V* p, const V* d, const V* v, V* out, V t) {
+    ++P->work[1];
+    sys_jvp<V>(P->sys, Y, p, d, v, out, t);
+}
+
+template <typename T, typename V>
+static void jacobian_diagonal_in(const Problem<T>* P, const V* Y, const V* p, const V* d, V t, V* diag) {
     const int n = P->n;
     for (int i = 0; i < n; ++i) {
-        T e[MAXN], jv[MAXN];
-        for (int k = 0; k < n; ++k) e[k] = T(k == i ? 1 : 0);
-        P->jvp(Y, params, drivers, e, jv, t);
+        V e[MAXN], jv[MAXN];
+        for (int k = 0; k < n; ++k) e[k] = V(k == i ? 1 : 0);
+        jvp_in<T, V>(P, Y, p, d, e, jv, t);
         diag[i] = jv[i];
     }
 }
+
+// Parameters / drivers rounded to the linear-solve precision V.
+template <typename T, typename V>
+struct InputsIn {
+    V p[MAXN], d[MAXS * 16];
+    void bind(const T* params, int np, const T* drivers, int nd) {
+        for (int i = 0; i < np; ++i) p[i] = V(params[i]);
+        for (int i = 0; i < nd; ++i) d[i] = V(drivers[i]);
+    }
+};
+
+// The linear-solver view of a problem in precision V: op / precond /
+// krylov_norm2 over V vectors, forwarded to the problem's *_in<V> methods.
+template <typename Prob, typename V>
+struct LinView {
+    const Prob* pb;
+    const OracleConfig* cfg;
+    void op(const V* v, V* out) const { pb->template op_in<V>(v, out); }
+    void neumann(const V* v, V* out) const { pb->template neumann_in<V>(v, out); }
+    void jacobi(const V* v, V* out) const { pb->template jacobi_in<V>(v, out); }
+    void precond(const V* v, V* out) const { precondition<V>(cfg, *this, v, out); }
+    V krylov_norm2(const V* v) const { return pb->template krylov_norm2_in<V>(v); }
+};
 
 // Jacobi step out = v / d with d = 1 - scale * J_ii, magnitude floored at 1e-16
 // (codegen/preconditioners.py:554-594 guarded division).
@@ -90,49 +119,66 @@ struct StageProblem {
     T katol[MAXN], krtol[MAXN], natol[MAXN], nrtol[MAXN];
     T inv_n;
 
-    void jvp(const T* v, T* out) const {
-        T Y[MAXN];
-        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
-        P->jvp(Y, params, drivers, v, out, t);
+    // Linear-solve callbacks in precision V (V = T except in the mixed mode).
+    template <typename V>
+    void stage_state_in(V* Y) const {
+        for (int i = 0; i < n; ++i) Y[i] = V(base[i] + a * u[i]);
     }
-    void op(const T* v, T* out) const {
-        T jv[MAXN];
-        jvp(v, jv);
-        T ah = a * h;
+    template <typename V>
+    void op_in(const V* v, V* out) const {
+        V Y[MAXN], jv[MAXN];
+        InputsIn<T, V> in;
+        in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
+        stage_state_in<V>(Y);
+        jvp_in<T, V>(P, Y, in.p, in.d, v, jv, V(t));
+        const V ah = V(a * h);
         for (int i = 0; i < n; ++i) out[i] = v[i] - ah * jv[i];
     }
-    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
-    void jacobi(const T* v, T* out) const {
-        T Y[MAXN], dg[MAXN];
-        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
-        jacobian_diagonal<T>(P, Y, params, drivers, t, dg);
-        const T scale = h * a;
-        for (int i = 0; i < n; ++i) out[i] = jacobi_divide<T>(v[i], scale, dg[i]);
+    template <typename V>
+    void jacobi_in(const V* v, V* out) const {
+        V Y[MAXN], dg[MAXN];
+        InputsIn<T, V> in;
+        in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
+        stage_state_in<V>(Y);
+        jacobian_diagonal_in<T, V>(P, Y, in.p, in.d, V(t), dg);
+        const V scale = V(h * a);
+        for (int i = 0; i < n; ++i) out[i] = jacobi_divide<V>(v[i], scale, dg[i]);
     }
-    void neumann(const T* v, T* out) const {
-        T h_eff = h * a;
+    template <typename V>
+    void neumann_in(const V* v, V* out) const {
+        const V h_eff = V(h * a);
+        V Y[MAXN];
+        InputsIn<T, V> in;
+        in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
+        stage_state_in<V>(Y);
         for (int i = 0; i < n; ++i) out[i] = v[i];
         for (int k = 0; k < cfg->preconditioner_order; ++k) {
-            T jv[MAXN];
-            jvp(out, jv);
+            V jv[MAXN];
+            jvp_in<T, V>(P, Y, in.p, in.d, out, jv, V(t));
             for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * jv[i];
         }
     }
+    template <typename V>
+    V krylov_norm2_in(const V* v) const {
+        V s = V(0);
+        for (int i = 0; i < n; ++i) {
+            V tol = V(katol[i]) + V(krtol[i]) * std::fabs(V(base[i]));
+            tol = std::max(tol, V(1e-16));
+            V r = std::fabs(v[i]) / tol;
+            s += r * r;
+        }
+        return s * V(inv_n);
+    }
+    void op(const T* v, T* out) const { op_in<T>(v, out); }
+    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
+    void jacobi(const T* v, T* out) const { jacobi_in<T>(v, out); }
+    void neumann(const T* v, T* out) const { neumann_in<T>(v, out); }
+    T krylov_norm2(const T* v) const { return krylov_norm2_in<T>(v); }
     void residual(T* out) const {
         T Y[MAXN], f[MAXN], obs[MAXN];
         for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
         P->rhs(Y, params, drivers, obs, f, t);
         for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
-    }
-    T krylov_norm2(const T* v) const {
-        T s = T(0);
-        for (int i = 0; i < n; ++i) {
-            T tol = katol[i] + krtol[i] * std::fabs(base[i]);
-            tol = std::max(tol, T(1e-16));
-            T r = std::fabs(v[i]) / tol;
-            s += r * r;
-        }
-        return s * inv_n;
     }
     T correction_norm2(const T* v) const {
         T s = T(0);
@@ -280,7 +326,17 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         Pb.residual(res);
         for (int i = 0; i < W; ++i) { res[i] = -res[i]; delta[i] = T(0); }
         int kit = 0;
-        int32_t lin = linear_solve<T>(cfg, Pb, W, res, delta, true, kit);
+        int32_t lin;
+        if (cfg->mixed) {
+            // FP32 Krylov solve of the FP64 Newton system
+            float rl[MAXW], dl[MAXW];
+            for (int i = 0; i < W; ++i) { rl[i] = float(res[i]); dl[i] = 0.0f; }
+            LinView<Prob, float> lv{&Pb, cfg};
+            lin = linear_solve<float>(cfg, lv, W, rl, dl, true, kit);
+            for (int i = 0; i < W; ++i) delta[i] = T(dl[i]);
+        } else {
+            lin = linear_solve<T>(cfg, Pb, W, res, delta, true, kit);
+        }
         ktot += kit;
         last_lin = lin;
         ++iters;
@@ -305,6 +361,8 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         prev_theta = (judged && hist) ? theta : prev_theta;
     }
     theta_store = conv ? prev_theta : T(1);
+    Pb.P->work[4] += ktot;
+    Pb.P->work[5] += iters;
     int32_t fb = failed ? ST_NEWTON_DIVERGENCE : ST_NEWTON_MAX_ITERS;
     if (last_lin != ST_SUCCESS) fb |= last_lin;
     counters[0] = iters;
@@ -555,39 +613,81 @@ struct FIRKProblem {
     T stage_time(int i) const { return t + h * c[i]; }
     const T* stage_drivers(int i) const { return drivers + i * nd; }
 
-    void op(const T* v, T* out) const {
-        for (int i = 0; i < s; ++i) {
-            T Y[MAXN], dir[MAXN], jv[MAXN];
-            combine(i, u, Y, true);
-            combine(i, v, dir, false);
-            P->jvp(Y, params, stage_drivers(i), dir, jv, stage_time(i));
-            for (int k = 0; k < n; ++k) out[i * n + k] = v[i * n + k] - h * jv[k];
+    // Linear-solve callbacks in precision V (V = T except in the mixed mode).
+    template <typename V>
+    void stage_state_in(int i, V* Y) const {
+        T Yr[MAXN];
+        combine(i, u, Yr, true);
+        for (int k = 0; k < n; ++k) Y[k] = V(Yr[k]);
+    }
+    template <typename V>
+    void combine_in(int i, const V* vec, V* out) const {
+        for (int k = 0; k < n; ++k) {
+            V acc = V(0);
+            for (int j = 0; j < s; ++j)
+                if (A[i][j] != T(0)) acc += V(A[i][j]) * vec[j * n + k];
+            out[k] = acc;
         }
     }
-    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
-    void jacobi(const T* v, T* out) const {
+    template <typename V>
+    void op_in(const V* v, V* out) const {
+        InputsIn<T, V> in;
+        in.bind(params, cfg->n_params, drivers, s * nd);
+        const V hv = V(h);
         for (int i = 0; i < s; ++i) {
-            T Y[MAXN], dg[MAXN];
-            combine(i, u, Y, true);
-            jacobian_diagonal<T>(P, Y, params, stage_drivers(i), stage_time(i), dg);
-            const T scale = h * A[i][i];
-            for (int k = 0; k < n; ++k) out[i * n + k] = jacobi_divide<T>(v[i * n + k], scale, dg[k]);
+            V Y[MAXN], dir[MAXN], jv[MAXN];
+            stage_state_in<V>(i, Y);
+            combine_in<V>(i, v, dir);
+            jvp_in<T, V>(P, Y, in.p, in.d + i * nd, dir, jv, V(stage_time(i)));
+            for (int k = 0; k < n; ++k) out[i * n + k] = v[i * n + k] - hv * jv[k];
         }
     }
-    void neumann(const T* v, T* out) const {
+    template <typename V>
+    void jacobi_in(const V* v, V* out) const {
+        InputsIn<T, V> in;
+        in.bind(params, cfg->n_params, drivers, s * nd);
+        for (int i = 0; i < s; ++i) {
+            V Y[MAXN], dg[MAXN];
+            stage_state_in<V>(i, Y);
+            jacobian_diagonal_in<T, V>(P, Y, in.p, in.d + i * nd, V(stage_time(i)), dg);
+            const V scale = V(h * A[i][i]);
+            for (int k = 0; k < n; ++k) out[i * n + k] = jacobi_divide<V>(v[i * n + k], scale, dg[k]);
+        }
+    }
+    template <typename V>
+    void neumann_in(const V* v, V* out) const {
+        InputsIn<T, V> in;
+        in.bind(params, cfg->n_params, drivers, s * nd);
         const int W = s * n;
+        const V hv = V(h);
         for (int e = 0; e < W; ++e) out[e] = v[e];
         for (int k = 0; k < cfg->preconditioner_order; ++k) {
-            T jvp[MAXW];
+            V jvp[MAXW];
             for (int i = 0; i < s; ++i) {
-                T Y[MAXN], dir[MAXN];
-                combine(i, u, Y, true);
-                combine(i, out, dir, false);
-                P->jvp(Y, params, stage_drivers(i), dir, jvp + i * n, stage_time(i));
+                V Y[MAXN], dir[MAXN];
+                stage_state_in<V>(i, Y);
+                combine_in<V>(i, out, dir);
+                jvp_in<T, V>(P, Y, in.p, in.d + i * nd, dir, jvp + i * n, V(stage_time(i)));
             }
-            for (int e = 0; e < W; ++e) out[e] = v[e] + h * jvp[e];
+            for (int e = 0; e < W; ++e) out[e] = v[e] + hv * jvp[e];
         }
     }
+    template <typename V>
+    V krylov_norm2_in(const V* v) const {
+        V acc = V(0);
+        for (int e = 0; e < s * n; ++e) {
+            V tol = V(katol[e]) + V(krtol[e]) * std::fabs(V(base[e % n]));
+            tol = std::max(tol, V(1e-16));
+            V r = std::fabs(v[e]) / tol;
+            acc += r * r;
+        }
+        return acc * V(inv_w);
+    }
+    void op(const T* v, T* out) const { op_in<T>(v, out); }
+    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
+    void jacobi(const T* v, T* out) const { jacobi_in<T>(v, out); }
+    void neumann(const T* v, T* out) const { neumann_in<T>(v, out); }
+    T krylov_norm2(const T* v) const { return krylov_norm2_in<T>(v); }
     void residual(T* out) const {
         for (int i = 0; i < s; ++i) {
             T Y[MAXN], f[MAXN], obs[MAXN];
@@ -595,16 +695,6 @@ struct FIRKProblem {
             P->rhs(Y, params, stage_drivers(i), obs, f, stage_time(i));
             for (int k = 0; k < n; ++k) out[i * n + k] = u[i * n + k] - h * f[k];
         }
-    }
-    T krylov_norm2(const T* v) const {
-        T acc = T(0);
-        for (int e = 0; e < s * n; ++e) {
-            T tol = katol[e] + krtol[e] * std::fabs(base[e % n]);
-            tol = std::max(tol, T(1e-16));
-            T r = std::fabs(v[e]) / tol;
-            acc += r * r;
-        }
-        return acc * inv_w;
     }
     T correction_norm2(const T* v) const {
         T acc = T(0);
@@ -781,38 +871,58 @@ struct RosenbrockProblem {
     T t, h, gamma;
     T katol[MAXN], krtol[MAXN];
     T inv_n;
-    void op(const T* v, T* out) const {
-        T jv[MAXN];
-        P->jvp(state, params, drivers, v, jv, t);
-        const T gh = gamma * h;
+    template <typename V>
+    void inputs_in(V* y, InputsIn<T, V>& in) const {
+        for (int i = 0; i < n; ++i) y[i] = V(state[i]);
+        in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
+    }
+    template <typename V>
+    void op_in(const V* v, V* out) const {
+        V y[MAXN], jv[MAXN];
+        InputsIn<T, V> in;
+        inputs_in<V>(y, in);
+        jvp_in<T, V>(P, y, in.p, in.d, v, jv, V(t));
+        const V gh = V(gamma * h);
         for (int i = 0; i < n; ++i) out[i] = v[i] - gh * jv[i];
     }
-    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
-    void jacobi(const T* v, T* out) const {
-        T dg[MAXN];
-        jacobian_diagonal<T>(P, state, params, drivers, t, dg);
-        const T scale = gamma * h;
-        for (int i = 0; i < n; ++i) out[i] = jacobi_divide<T>(v[i], scale, dg[i]);
+    template <typename V>
+    void jacobi_in(const V* v, V* out) const {
+        V y[MAXN], dg[MAXN];
+        InputsIn<T, V> in;
+        inputs_in<V>(y, in);
+        jacobian_diagonal_in<T, V>(P, y, in.p, in.d, V(t), dg);
+        const V scale = V(gamma * h);
+        for (int i = 0; i < n; ++i) out[i] = jacobi_divide<V>(v[i], scale, dg[i]);
     }
-    void neumann(const T* v, T* out) const {
-        const T h_eff = h * gamma;
+    template <typename V>
+    void neumann_in(const V* v, V* out) const {
+        V y[MAXN];
+        InputsIn<T, V> in;
+        inputs_in<V>(y, in);
+        const V h_eff = V(h * gamma);
         for (int i = 0; i < n; ++i) out[i] = v[i];
         for (int k = 0; k < cfg->preconditioner_order; ++k) {
-            T jv[MAXN];
-            P->jvp(state, params, drivers, out, jv, t);
+            V jv[MAXN];
+            jvp_in<T, V>(P, y, in.p, in.d, out, jv, V(t));
             for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * jv[i];
         }
     }
-    T krylov_norm2(const T* v) const {
-        T acc = T(0);
+    template <typename V>
+    V krylov_norm2_in(const V* v) const {
+        V acc = V(0);
         for (int i = 0; i < n; ++i) {
-            T tol = katol[i] + krtol[i] * std::fabs(state[i]);
-            tol = std::max(tol, T(1e-16));
-            T r = std::fabs(v[i]) / tol;
+            V tol = V(katol[i]) + V(krtol[i]) * std::fabs(V(state[i]));
+            tol = std::max(tol, V(1e-16));
+            V r = std::fabs(v[i]) / tol;
             acc += r * r;
         }
-        return acc * inv_n;
+        return acc * V(inv_n);
     }
+    void op(const T* v, T* out) const { op_in<T>(v, out); }
+    void precond(const T* v, T* out) const { precondition<T>(cfg, *this, v, out); }
+    void jacobi(const T* v, T* out) const { jacobi_in<T>(v, out); }
+    void neumann(const T* v, T* out) const { neumann_in<T>(v, out); }
+    T krylov_norm2(const T* v) const { return krylov_norm2_in<T>(v); }
 };
 
 template <typename T>
@@ -856,6 +966,19 @@ struct RosenbrockStepper {
         }
     }
 
+    // (I - gamma h J) x = rhs from the guess in x; FP32 in the mixed mode.
+    int32_t solve(RosenbrockProblem<T>& rp, T* rhs, T* x, int& it) {
+        if (cfg->mixed) {
+            float rl[MAXN], xl[MAXN];
+            for (int i = 0; i < n; ++i) { rl[i] = float(rhs[i]); xl[i] = float(x[i]); }
+            LinView<RosenbrockProblem<T>, float> lv{&rp, cfg};
+            int32_t st = linear_solve<float>(cfg, lv, n, rl, xl, false, it);
+            for (int i = 0; i < n; ++i) x[i] = T(xl[i]);
+            return st;
+        }
+        return linear_solve<T>(cfg, rp, n, rhs, x, false, it);
+    }
+
     void time_derivative(const T* state, const T* params, const T* drivers, T* ddt, T* out,
                          T current_time, T dt) const {
         if (nd > 0) P->drivers_dt(current_time, ddt);
@@ -896,8 +1019,9 @@ struct RosenbrockStepper {
         }
         int it = 0;
         rp.t = current_time;   // J is taken at the step start
-        status |= linear_solve<T>(cfg, rp, n, rhs, stage_increment, false, it);
+        status |= solve(rp, rhs, stage_increment, it);
         krylov_total += it;
+        P->work[4] += it;
         for (int i = 0; i < n; ++i) store[i] = stage_increment[i];
         for (int i = 0; i < n; ++i) {
             if (accumulates_output) proposed[i] += stage_increment[i] * b[0];
@@ -926,8 +1050,9 @@ struct RosenbrockStepper {
                 rhs[i] = v * dt * gamma;
             }
             for (int i = 0; i < n; ++i) stage_increment[i] = store[(si - 1) * n + i];
-            status |= linear_solve<T>(cfg, rp, n, rhs, stage_increment, false, it);
+            status |= solve(rp, rhs, stage_increment, it);
             krylov_total += it;
+            P->work[4] += it;
             for (int i = 0; i < n; ++i) store[si * n + i] = stage_increment[i];
             for (int i = 0; i < n; ++i) {
                 if (accumulates_output) proposed[i] += b[si] * stage_increment[i];

@@ -52,13 +52,29 @@ static const double HH_EK = -77.0;
 static const double HH_EL = -54.387;
 static const double HH_CM = 1.0;
 
+// x / (1 - exp(-x/10)) has a removable singularity at x = 0 (value 10); within
+// |x| < 1e-3 the rates use its first-order expansion 10 + x/2, so a stage state
+// that lands exactly on V = -40 or V = -55 (likely in FP32) stays finite.
+static const double HH_GUARD = 1e-3;
+template <typename T>
+inline T hh_xratio(T x) {
+    return std::fabs(x) < T(HH_GUARD) ? T(10) + x / T(2) : x / (T(1) - std::exp(-x / T(10)));
+}
+template <typename T>
+inline T hh_dxratio(T x) {   // d/dx of hh_xratio
+    if (std::fabs(x) < T(HH_GUARD)) return T(0.5);
+    T e = std::exp(-x / T(10));
+    T den = T(1) - e;
+    return (den - x * e / T(10)) / (den * den);
+}
+
 template <typename T>
 inline void hh_rates(T V, T& am, T& bm, T& ah, T& bh, T& an, T& bn) {
-    am = T(0.1) * (V + T(40.0)) / (T(1.0) - std::exp(-(V + T(40.0)) / T(10.0)));
+    am = T(0.1) * hh_xratio(V + T(40.0));
     bm = T(4.0) * std::exp(-(V + T(65.0)) / T(18.0));
     ah = T(0.07) * std::exp(-(V + T(65.0)) / T(20.0));
     bh = T(1.0) / (T(1.0) + std::exp(-(V + T(35.0)) / T(10.0)));
-    an = T(0.01) * (V + T(55.0)) / (T(1.0) - std::exp(-(V + T(55.0)) / T(10.0)));
+    an = T(0.01) * hh_xratio(V + T(55.0));
     bn = T(0.125) * std::exp(-(V + T(65.0)) / T(80.0));
 }
 
@@ -186,15 +202,8 @@ void sys_jvp(const SystemDesc& s, const T* y, const T* p, const T* d,
         const T V = y[0], m = y[1], h = y[2], n = y[3];
         T am, bm, ah, bh, an, bn;
         hh_rates<T>(V, am, bm, ah, bh, an, bn);
-        // d/dV of x/(1-exp(-x/10)) scaled forms
-        auto dratio = [](T a, T x, T scale) {
-            // a = scale * x / (1 - e), e = exp(-x/10)
-            T e = std::exp(-x / T(10));
-            T den = T(1) - e;
-            return scale * (den - x * e / T(10)) / (den * den);
-        };
-        const T dam = dratio(am, V + T(40), T(0.1));
-        const T dan = dratio(an, V + T(55), T(0.01));
+        const T dam = T(0.1) * hh_dxratio(V + T(40));
+        const T dan = T(0.01) * hh_dxratio(V + T(55));
         const T dbm = -bm / T(18);
         const T dah = -ah / T(20);
         const T ebh = std::exp(-(V + T(35)) / T(10));

@@ -6,12 +6,15 @@ import cubie_amd as ca
 
 LORENZ = ["dx = sigma*(y - x)", "dy = x*(rho - z) - y", "dz = x*y - beta*z"]
 
+# alpha_m and alpha_n have removable singularities at V = -40 / -55 mV; within
+# 1e-3 mV of them the first-order expansion of x / (1 - exp(-x/10)) = 10 + x/2
+# keeps every evaluation finite (an FP32 stage state can land exactly on them).
 HODGKIN_HUXLEY = """
-alpha_m = 0.1*(V + 40)/(1 - exp(-(V + 40)/10))
+alpha_m = Piecewise((1 + (V + 40)/20, Abs(V + 40) < 1e-3), (0.1*(V + 40)/(1 - exp(-(V + 40)/10)), True))
 beta_m = 4*exp(-(V + 65)/18)
 alpha_h = 0.07*exp(-(V + 65)/20)
 beta_h = 1/(1 + exp(-(V + 35)/10))
-alpha_n = 0.01*(V + 55)/(1 - exp(-(V + 55)/10))
+alpha_n = Piecewise((0.1 + (V + 55)/200, Abs(V + 55) < 1e-3), (0.01*(V + 55)/(1 - exp(-(V + 55)/10)), True))
 beta_n = 0.125*exp(-(V + 65)/80)
 dV = (I_ext - gNa*m**3*h*(V - ENa) - gK*n**4*(V - EK) - gL*(V - EL))/Cm
 dm = alpha_m*(1 - m) - beta_m*m

@@ -1,0 +1,123 @@
+"""The compensated (mixed-precision) mode against the FP64 oracle.
+
+Explicit families: FP32 stages, a Kahan-compensated FP32 state update and
+the FP64 error norm. Implicit families: FP64 outer iteration (state, stage
+values, Newton residual, iterate and norms) with FP32 Krylov solves. Each
+result is compared with the oracle integrating the same inputs in FP64 at
+the same settings; bars are stated per test. Plain FP32 fails the first
+test outright (its Newton iteration cannot meet a 1e-7 relative tolerance in
+single precision), which is what the mode is for.
+"""
+
+import numpy as np
+import pytest
+
+import cubie_amd as ca
+
+from systems import (hh_batch, hh_driver, hodgkin_huxley, lorenz, lorenz_batch, normwise_rel,
+                     robertson, robertson_batch, van_der_pol)
+
+pytestmark = pytest.mark.gpu
+
+
+def fp64_oracle(oracle_module, system64, solver_kw, system_id, inits, params, duration, drivers=None):
+    ref_solver = ca.Solver(system64, **solver_kw)
+    if drivers is not None:
+        ref_solver._configure_drivers(drivers)
+    run = oracle_module.OracleRun(ref_solver.kernel, system_id, duration)
+    return run.solve(inits, params, threads=8)
+
+
+def test_hh_sdirk3_compensated_meets_fp64(oracle_module):
+    """HH, L-stable SDIRK3 at dt 2^-6 ms with BiCGSTAB, atol = rtol = 1e-6,
+    256 systems over 100 ms. Every run succeeds; the summaries (mean V,
+    max V, spike count) agree with the FP64 oracle at rtol 1e-12 (same FP32
+    inputs) to 1e-6 relative, spike counts exactly; the trajectories agree
+    with the oracle's own mixed-precision restatement to 1e-6 normwise."""
+    n = 256
+    inits, params = hh_batch(n, seed=41)
+    inits = inits.astype(np.float32).astype(np.float64)
+    params = params.astype(np.float32).astype(np.float64)
+    kw = dict(algorithm="l_stable_dirk_3", dt=2.0 ** -6, atol=1e-6, rtol=1e-6,
+              linear_correction_type="bicgstab", summarise_every=50.0,
+              sample_summaries_every=2.0 ** -4, summarise_variables=["V"],
+              save_variables=["V"], save_every=1.0,
+              output_types=["state", "mean", "max", "spike_count[0]"])
+    solver = ca.Solver(hodgkin_huxley(np.float32), precision_mode="compensated", **kw)
+    solver._configure_drivers(hh_driver())
+    res = solver.solve(inits, params, duration=100.0, nan_error_trajectories=False)
+    assert np.all(res.status_codes == 0), res.status_messages
+    tight = dict(kw, atol=1e-12, rtol=1e-12)
+    ref = fp64_oracle(oracle_module, hodgkin_huxley(np.float64), tight, "hodgkin_huxley",
+                      inits, params, 100.0, hh_driver())
+    s, r = res.state_summaries.astype(np.float64), ref["state_summaries"]
+    assert np.array_equal(s[:, 2], r[:, 2])
+    errs = [np.abs(s[:, k] - r[:, k]).max() / np.abs(r[:, k]).max() for k in (0, 1)]
+    print("hh compensated summaries vs fp64 (rtol 1e-12): mean, max", errs)
+    assert max(errs) <= 1e-6
+    mixed = oracle_module.OracleRun(solver.kernel, "hodgkin_huxley", 100.0).solve(inits, params, threads=8)
+    err = normwise_rel(res.state.astype(np.float64), mixed["state"])
+    print("hh compensated vs mixed oracle normwise", err)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("alg", ["dormand-prince-54", "tsit5"])
+def test_van_der_pol_compensated_erk(oracle_module, alg):
+    """Explicit compensated mode on van der Pol (mu in [0.5, 2]), rtol 1e-7:
+    within 1e-5 normwise of the FP64 oracle, and closer to it than plain FP32."""
+    n = 64
+    rng = np.random.default_rng(43)
+    inits = rng.uniform(-2.0, 2.0, (2, n))
+    params = rng.uniform(0.5, 2.0, (1, n))
+    kw = dict(algorithm=alg, atol=1e-9, rtol=1e-7, save_every=0.5,
+              output_types=["state", "iteration_counters"])
+    comp = ca.Solver(van_der_pol(np.float32), precision_mode="compensated", **kw)
+    plain = ca.Solver(van_der_pol(np.float32), **kw)
+    rc = comp.solve(inits, params, duration=20.0, nan_error_trajectories=False)
+    rp = plain.solve(inits, params, duration=20.0, nan_error_trajectories=False)
+    assert np.all(rc.status_codes == 0)
+    ref = fp64_oracle(oracle_module, van_der_pol(np.float64), kw, "van_der_pol", inits, params, 20.0)
+    ec = normwise_rel(rc.state.astype(np.float64), ref["state"])
+    ep = normwise_rel(rp.state.astype(np.float64), ref["state"])
+    print(alg, "compensated", ec, "plain fp32", ep)
+    assert ec <= 1e-5
+    assert ec <= ep
+
+
+@pytest.mark.parametrize("alg,bar", [("radau", 1e-5), ("ros3p", 5e-5), ("kvaerno5", 1e-5)])
+def test_robertson_compensated(oracle_module, alg, bar):
+    """Stiff kinetics in the mixed mode, against FP64. Newton methods correct
+    their FP32 Krylov solutions against an FP64 residual (1e-5); a
+    Rosenbrock stage is one FP32 linear solve with no outer correction, so
+    its stages carry FP32 solve error (5e-5)."""
+    n = 64
+    inits, params = robertson_batch(n, seed=45)
+    kw = dict(algorithm=alg, atol=1e-10, rtol=1e-6, linear_correction_type="bicgstab",
+              save_every=0.5, output_types=["state", "iteration_counters"])
+    solver = ca.Solver(robertson(np.float32), precision_mode="compensated", **kw)
+    res = solver.solve(inits, params, duration=10.0, nan_error_trajectories=False)
+    assert np.all(res.status_codes == 0), res.status_messages
+    ref = fp64_oracle(oracle_module, robertson(np.float64), kw, "robertson", inits, params, 10.0)
+    err = normwise_rel(res.state.astype(np.float64), ref["state"])
+    print(alg, "robertson compensated vs fp64", err)
+    assert err <= bar
+    mixed = oracle_module.OracleRun(solver.kernel, "robertson", 10.0).solve(inits, params, threads=8)
+    assert np.array_equal(res.status_codes, mixed["status_codes"])
+    err_m = normwise_rel(res.state.astype(np.float64), mixed["state"])
+    print(alg, "robertson compensated vs mixed oracle", err_m)
+    assert err_m <= bar
+
+
+def test_lorenz_compensated_persistent_matches_plain_kernel():
+    """The persistent work-queue kernel returns exactly what the one-lane-
+    per-system kernel does, in the compensated mode."""
+    n = 300     # not a multiple of the block size: the queue hands out a ragged tail
+    inits, params = lorenz_batch(n, seed=47)
+    kw = dict(algorithm="dormand-prince-54", atol=1e-8, rtol=1e-6, save_every=0.1,
+              output_types=["state", "iteration_counters"])
+    a = ca.Solver(lorenz(np.float32), precision_mode="compensated", kernel_mode="plain", **kw)
+    b = ca.Solver(lorenz(np.float32), precision_mode="compensated", kernel_mode="persistent", **kw)
+    ra = a.solve(inits, params, duration=2.0, nan_error_trajectories=False)
+    rb = b.solve(inits, params, duration=2.0, nan_error_trajectories=False)
+    assert np.array_equal(ra.state, rb.state)
+    assert np.array_equal(ra.iteration_counters, rb.iteration_counters)
