@@ -56,13 +56,38 @@ struct FirkProblem {
         for (int k = 0; k < n; ++k) Y[k] = L(Yr[k]);
     }
 
+    static constexpr bool cache = Cfg::cache_jac;
+    Vec<L, (cache ? s * n * n : 1)> J;   // J(Y_i) per stage, refreshed per linear solve
+
+    // Called by the Newton iteration after each residual (fixed iterate
+    // during the Krylov solve that follows).
+    __device__ __forceinline__ void prepare_linear() {
+        if constexpr (cache) {
+#pragma unroll
+            for (int i = 0; i < s; ++i) {
+                L Y[n];
+                stage_state_l(i, Y);
+                Sys::jac(Y, params_l, drivers_l + i * nd, J.data() + i * n * n, L(stage_time(i)));
+            }
+        }
+    }
+    // J(Y_i) dir for stage i
+    __device__ __forceinline__ void stage_jv(int i, const L* dir, L* out) const {
+        if constexpr (cache) {
+            dense_jv<Sys, n>(J.data() + i * n * n, dir, out);
+        } else {
+            L Y[n];
+            stage_state_l(i, Y);
+            Sys::jvp(Y, params_l, drivers_l + i * nd, dir, out, L(stage_time(i)));
+        }
+    }
+
     __device__ __forceinline__ void op(const L* v, L* out) const {
 #pragma unroll
         for (int i = 0; i < s; ++i) {
-            L Y[n], dir[n], jv[n];
-            stage_state_l(i, Y);
+            L dir[n], jv[n];
             combine<false>(i, v, dir);
-            Sys::jvp(Y, params_l, drivers_l + i * nd, dir, jv, L(stage_time(i)));
+            stage_jv(i, dir, jv);
             const L hl = L(h);
 #pragma unroll
             for (int k = 0; k < n; ++k) out[i * n + k] = v[i * n + k] - hl * jv[k];
@@ -76,10 +101,9 @@ struct FirkProblem {
             L jvp[W];
 #pragma unroll
             for (int i = 0; i < s; ++i) {
-                L Y[n], dir[n];
-                stage_state_l(i, Y);
+                L dir[n];
                 combine<false>(i, out, dir);
-                Sys::jvp(Y, params_l, drivers_l + i * nd, dir, jvp + i * n, L(stage_time(i)));
+                stage_jv(i, dir, jvp + i * n);
             }
 #pragma unroll
             for (int e = 0; e < W; ++e) out[e] = v[e] + hl * jvp[e];
@@ -88,9 +112,15 @@ struct FirkProblem {
     __device__ __forceinline__ void jacobi(const L* v, L* out) const {
 #pragma unroll
         for (int i = 0; i < s; ++i) {
-            L Y[n], jd[n];
-            stage_state_l(i, Y);
-            Sys::jac_diag(Y, params_l, drivers_l + i * nd, jd, L(stage_time(i)));
+            L jd[n];
+            if constexpr (cache) {
+#pragma unroll
+                for (int k = 0; k < n; ++k) jd[k] = J[i * n * n + k * n + k];
+            } else {
+                L Y[n];
+                stage_state_l(i, Y);
+                Sys::jac_diag(Y, params_l, drivers_l + i * nd, jd, L(stage_time(i)));
+            }
             const L scale = L(h * Cfg::A[i][i]);
 #pragma unroll
             for (int k = 0; k < n; ++k) out[i * n + k] = jacobi_divide(v[i * n + k], scale, jd[k]);

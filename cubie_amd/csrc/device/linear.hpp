@@ -106,6 +106,19 @@ __device__ __forceinline__ T jacobi_divide(T v, T scale, T jii) {
     return v / safe;
 }
 
+// out = J v with the dense row-major Jacobian, skipping structural zeros.
+template <typename Sys, int n, typename L>
+__device__ __forceinline__ void dense_jv(const L* J, const L* v, L* out) {
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+        L acc = L(0);
+#pragma unroll
+        for (int j = 0; j < n; ++j)
+            if (Sys::jac_nz[i * n + j]) acc += J[i * n + j] * v[j];
+        out[i] = acc;
+    }
+}
+
 // Preconditioned minimal-residual / steepest-descent solve of A x = rhs.
 // rhs is overwritten by the running residual; x holds the initial guess.
 template <typename Cfg, typename L, typename Prob>
@@ -284,6 +297,7 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         if (converged || failed) break;
         ++iteration;
         P.residual(residual);
+        P.prepare_linear();
         int kit = 0;
         int lin;
         if constexpr (same_type<L, real>::value) {
@@ -375,37 +389,61 @@ struct DirkStageProblem {
     const L* drivers_l;
     real t, h, a;
 
+    static constexpr bool cache = Cfg::cache_jac;
+    Vec<L, (cache ? n * n : 1)> J;   // J(base + a u), refreshed per linear solve
+
     __device__ __forceinline__ void stage_state_l(L* Y) const {
 #pragma unroll
         for (int i = 0; i < n; ++i) Y[i] = L(base[i] + a * u[i]);
     }
+    // Called by the Newton iteration after each residual: the Krylov solve
+    // that follows sees a fixed iterate, hence a fixed Jacobian.
+    __device__ __forceinline__ void prepare_linear() {
+        if constexpr (cache) {
+            L Y[n];
+            stage_state_l(Y);
+            Sys::jac(Y, params_l, drivers_l, J.data(), L(t));
+        }
+    }
+    __device__ __forceinline__ void jv(const L* v, L* out) const {
+        if constexpr (cache) {
+            dense_jv<Sys, n>(J.data(), v, out);
+        } else {
+            L Y[n];
+            stage_state_l(Y);
+            Sys::jvp(Y, params_l, drivers_l, v, out, L(t));
+        }
+    }
     __device__ __forceinline__ void op(const L* v, L* out) const {
-        L Y[n], jv[n];
-        stage_state_l(Y);
-        Sys::jvp(Y, params_l, drivers_l, v, jv, L(t));
+        L w[n];
+        jv(v, w);
         const L ah = L(a * h);
 #pragma unroll
-        for (int i = 0; i < n; ++i) out[i] = v[i] - ah * jv[i];
+        for (int i = 0; i < n; ++i) out[i] = v[i] - ah * w[i];
     }
     __device__ __forceinline__ void jacobi(const L* v, L* out) const {
-        L Y[n], jd[n];
-        stage_state_l(Y);
-        Sys::jac_diag(Y, params_l, drivers_l, jd, L(t));
+        L jd[n];
+        if constexpr (cache) {
+#pragma unroll
+            for (int i = 0; i < n; ++i) jd[i] = J[i * n + i];
+        } else {
+            L Y[n];
+            stage_state_l(Y);
+            Sys::jac_diag(Y, params_l, drivers_l, jd, L(t));
+        }
         const L scale = L(h * a);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = jacobi_divide(v[i], scale, jd[i]);
     }
     __device__ __forceinline__ void neumann(const L* v, L* out) const {
         const L h_eff = L(h * a);
-        L Y[n];
-        stage_state_l(Y);
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = v[i];
         for (int k = 0; k < Cfg::preconditioner_order; ++k) {
-            L jv[n];
-            Sys::jvp(Y, params_l, drivers_l, out, jv, L(t));
+            L w[n];
+            jv(out, w);
 #pragma unroll
-            for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * jv[i];
+            for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * w[i];
         }
     }
     __device__ __forceinline__ void residual(real* out) const {

@@ -24,7 +24,7 @@ struct RosenbrockProblem {
     using L = typename Cfg::lin_real;
     static constexpr int n = Cfg::n;
     static constexpr int W = n;
-    static constexpr bool cache_jac = Cfg::n <= 8;
+    static constexpr bool cache_jac = Cfg::cache_jac;
     const L* state;     // y_n, in the linear-solve precision
     const L* params;
     const L* drivers;
@@ -32,15 +32,10 @@ struct RosenbrockProblem {
     const L* Jd;        // Jacobian diagonal
     real t, h;
 
+    __device__ __forceinline__ void prepare_linear() {}   // J is taken once per step
     __device__ __forceinline__ void jv(const L* v, L* out) const {
         if constexpr (cache_jac) {
-#pragma unroll
-            for (int i = 0; i < n; ++i) {
-                L acc = L(0);
-#pragma unroll
-                for (int j = 0; j < n; ++j) acc += J[i * n + j] * v[j];
-                out[i] = acc;
-            }
+            dense_jv<Sys, n>(J, v, out);
         } else {
             Sys::jvp(state, params, drivers, v, out, L(t));
         }

@@ -78,6 +78,20 @@ class ImplicitSettings:
             raise ValueError("preconditioner_type must name at least one preconditioner")
         return kinds
 
+    def krylov_stopping(self, precision, mixed: bool = False) -> Tuple[float, float]:
+        """(reduction, floor) of the Krylov stopping test
+        ||r|| <= floor + reduction ||b|| (linear_solver_base.py:59-140):
+        unset values default to eps and sqrt(eps) of the precision. In the
+        mixed-precision mode the Krylov iterations run in FP32, so the
+        reduction is floored at 8 eps(FP32): a Rosenbrock default of
+        rtol / 100 = 1e-8 is unreachable in single precision."""
+        eps = float(np.finfo(precision).eps)
+        red = self.krylov_residual_reduction if self.krylov_residual_reduction is not None else eps
+        flo = self.krylov_residual_floor if self.krylov_residual_floor is not None else float(np.sqrt(eps))
+        if mixed:
+            red = max(red, 8.0 * float(np.finfo(np.float32).eps))
+        return float(red), float(flo)
+
     def tol(self, name: str, n: int) -> np.ndarray:
         arr = np.atleast_1d(np.asarray(getattr(self, name), dtype=np.float64))
         if arr.size == 1:
@@ -254,6 +268,12 @@ class KernelSpec:
         a(f"  static constexpr int krylov_max_iters = {int(im.krylov_max_iters)};")
         a(f"  static constexpr int linear_solver = {lin};")
         a(f"  static constexpr int preconditioner_order = {int(im.preconditioner_order)};")
+        # Small systems evaluate the dense Jacobian once per linear solve and
+        # apply it as a matrix-vector product (registers: n^2, or s n^2 for the
+        # coupled FIRK stages), instead of re-deriving J v from the
+        # right-hand side for every Krylov vector.
+        jac_regs = n * n * (st if alg.family == "firk" else 1)
+        a(f"  static constexpr bool cache_jac = {str(jac_regs <= 64).lower()};")
         kinds = [PRECONDITIONER_KINDS[k] for k in im.preconditioner_kinds]
         a(f"  static constexpr int precond_count = {len(kinds)};")
         a(f"  static constexpr int precond_kinds[{len(kinds)}] = {c_int_array(kinds)};")
@@ -285,8 +305,7 @@ class KernelSpec:
         diag = [tab.a[i][i] for i in range(st)]
         later_explicit = not all(x != 0.0 for x in diag[1:])
         a(f"  static constexpr bool dirk_has_later_explicit = {str(later_explicit).lower()};")
-        red = im.krylov_residual_reduction if im.krylov_residual_reduction is not None else float(np.finfo(p).eps)
-        flo = im.krylov_residual_floor if im.krylov_residual_floor is not None else float(np.sqrt(np.finfo(p).eps))
+        red, flo = im.krylov_stopping(p, self.mixed_implicit)
         a(f"  static constexpr real krylov_reduction = {self._lit(red)};")
         a(f"  static constexpr real krylov_floor = {self._lit(flo)};")
         a("};")

@@ -11,7 +11,8 @@ emitted as HIP C++ for hiprtc:
                                    contracted with v (no dense J in memory)
 * ``dfdt(y, p, d, dd, out, t)``    df/dt with driver derivatives dd
 * ``jac_diag(y, p, d, Jd, t)``     diagonal of J (Jacobi preconditioner)
-* ``jac(y, p, d, J, t)``           dense J (row-major), for the adjoint
+* ``jac(y, p, d, J, t)``           dense J (row-major) and its structural
+                                   nonzero mask ``jac_nz``
 * ``jacp(y, p, d, Jp, t)``         dense df/dparams (row-major)
 
 Every function is a template on the scalar type ``real`` (the working
@@ -316,8 +317,10 @@ class SystemCodegen:
         parts += self._emit_block(Jd, "jac_diag")
         parts.append("    (void)y; (void)p; (void)d; (void)Jd; (void)t;")
         parts.append("  }")
-        # dense Jacobians for the adjoint
+        # dense Jacobian (cached per linear solve for small systems, and the adjoint)
         J = [(f"J[{i * n + j}]", sp.diff(dx[i], s.state_symbols[j])) for i in range(n) for j in range(n)]
+        nz = ", ".join("true" if not e.is_zero else "false" for _, e in J)
+        parts.append(f"  static constexpr bool jac_nz[{max(n * n, 1)}] = {{{nz or 'false'}}};")
         parts.append("  template <typename real> __device__ static __forceinline__ void jac(const real* __restrict__ y, "
                      "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ J, real t) {")
         parts += self._bindings()

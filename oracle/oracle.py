@@ -315,11 +315,7 @@ class OracleRun:
         cfg.newton_rtol_s = float(np.min(im.newton_rtol))
         cfg.krylov_atol_s = float(np.min(im.krylov_atol))
         cfg.krylov_rtol_s = float(np.min(im.krylov_rtol))
-        cfg.krylov_residual_reduction = (im.krylov_residual_reduction
-                                         if im.krylov_residual_reduction is not None
-                                         else float(np.finfo(prec).eps))
-        cfg.krylov_floor = (im.krylov_residual_floor if im.krylov_residual_floor is not None
-                            else float(np.sqrt(np.finfo(prec).eps)))
+        cfg.krylov_residual_reduction, cfg.krylov_floor = im.krylov_stopping(prec, self.mixed)
         if callbacks is not None:
             cfg.rhs_cb, cfg.obs_cb, cfg.jvp_cb = callbacks[:3]
             cfg.dfdt_cb = callbacks[3] if len(callbacks) > 3 else DFDT_CB()
