@@ -13,7 +13,11 @@ succeed); every variant's rate, its speed-up over the FP64 plain kernel and
 its summary error against it are reported under config.variants.
 
 Default workload: the 1,048,576-system Hodgkin-Huxley batch (L-stable
-SDIRK3 + BiCGSTAB at dt 0.01 ms over 500 ms, summaries only).
+SDIRK3 + BiCGSTAB at dt 2^-6 ms over 500 ms, summaries only), run as the
+FP32-compensated persistent kernel and the FP64 plain kernel. Before it is
+timed, each variant's launch shape (occupancy floor, and for the persistent
+kernel the workgroups per CU) is picked by the autotuner on a 262,144-system
+sample of the same batch.
 
 Prints one JSON line (see the repository's benchmark contract).
 """
@@ -101,11 +105,14 @@ def hh_workload(n, precision=np.float64, duration=500.0, dt=2.0 ** -6):
     0.01 the accumulated time drifts against the event schedule and the
     loop's positive-gap clamp inserts ~0.8 extra micro-steps per event."""
     rng = np.random.default_rng(2024)
-    params = np.array([120.0, 36.0, 0.3])[:, None] * rng.uniform(0.7, 1.3, (3, n))
-    inits = np.tile(np.array([[-65.0], [0.0529], [0.5961], [0.3177]]), (1, n))
+    # Inputs are rounded to FP32-representable values, so the FP32 and FP64
+    # kernels integrate the same problem and their summaries can be compared.
+    f32 = lambda x: np.asarray(x, dtype=np.float32).astype(np.float64)
+    params = f32(np.array([120.0, 36.0, 0.3])[:, None] * rng.uniform(0.7, 1.3, (3, n)))
+    inits = f32(np.tile(np.array([[-65.0], [0.0529], [0.5961], [0.3177]]), (1, n)))
     drng = np.random.default_rng(7)
     t = np.arange(1000) * 0.5
-    current = 10.0 * np.sin(2.0 * np.pi * t / 50.0) + drng.normal(0.0, 2.0, 1000)
+    current = f32(10.0 * np.sin(2.0 * np.pi * t / 50.0) + drng.normal(0.0, 2.0, 1000))
     solver_kw = dict(algorithm="l_stable_dirk_3", dt=dt, atol=1e-6, rtol=1e-6,
                      linear_correction_type="bicgstab", save_every=duration,
                      summarise_every=duration, sample_summaries_every=4 * dt,
@@ -115,12 +122,45 @@ def hh_workload(n, precision=np.float64, duration=500.0, dt=2.0 ** -6):
             "inits": inits, "params": params,
             "duration": duration, "solver_kw": solver_kw, "oracle_system": "hodgkin_huxley",
             "drivers": {"I_ext": current, "dt": 0.5, "order": 1, "wrap": False},
+            "summary_columns": ["mean_V", "max_V", "spike_count_V"],
+            "f32_problem": True,
             "config": {"workload": f"Hodgkin-Huxley L-stable SDIRK3 + BiCGSTAB, dt {dt:g} ms, "
                                    f"t in [0,{duration:g}] ms, summaries mean/max/spike count of V",
                        "n_systems": n}}
 
 
-WORKLOADS = {"lorenz": lorenz_workload, "hh": hh_workload}
+def vdp_system(precision):
+    import cubie_amd as ca
+    return ca.create_ODE_system(
+        ["dx = y", "dy = mu*(1 - x**2)*y - x"], states={"x": 2.0, "y": 0.0},
+        parameters={"mu": 1.0}, precision=precision, name="van_der_pol")
+
+
+def vdp_workload(n, precision=np.float64, duration=100.0):
+    """Uneven Van der Pol batch: mu log-uniform on [0.1, 1000], end time
+    uniform on [1, 100], y0 uniform on [-2, 2]^2 and a per-system rtol
+    log-uniform on [1e-8, 1e-4] (atol 1e-10); DOPRI5 with the PI controller;
+    only the final state is kept. The work per system spans about four
+    orders of magnitude, the case the persistent work queue is for."""
+    rng = np.random.default_rng(4)
+    f32 = lambda x: np.asarray(x, dtype=np.float32).astype(np.float64)
+    mu = f32(np.exp(rng.uniform(np.log(0.1), np.log(1000.0), n)))[None, :]
+    t_end = f32(rng.uniform(1.0, 100.0, n))
+    inits = f32(rng.uniform(-2.0, 2.0, (2, n)))
+    rtol = f32(np.exp(rng.uniform(np.log(1e-8), np.log(1e-4), n)))
+    solver_kw = dict(algorithm="dormand-prince-54", atol=1e-10, rtol=1e-6, dt_min=1e-10,
+                     dt_max=1.0, output_types=["state", "iteration_counters"])
+    return {"name": "vdp_uneven_dopri5", "system_factory": vdp_system,
+            "inits": inits, "params": mu, "duration": float(duration), "solver_kw": solver_kw,
+            "oracle_system": "van_der_pol",
+            "run_kw": {"rtol_per_run": rtol, "duration_per_run": t_end},
+            "config": {"workload": "Van der Pol uneven batch, DOPRI5, mu log-uniform [0.1,1000], "
+                                   "t_end U[1,100], y0 U[-2,2]^2, rtol log-uniform [1e-8,1e-4] "
+                                   "per system, atol 1e-10, final state",
+                       "n_systems": n}}
+
+
+WORKLOADS = {"lorenz": lorenz_workload, "hh": hh_workload, "vdp": vdp_workload}
 
 
 # ---------------------------------------------------------------------------
@@ -185,13 +225,16 @@ def run_oracle(work, kernel, n_sample, threads):
     import oracle as orc
     orc.build()
     run = orc.OracleRun(kernel, work["oracle_system"], work["duration"])
+    run_kw = {k: np.asarray(v)[:n_sample] for k, v in work.get("run_kw", {}).items()}
     t = time.perf_counter()
-    res = run.solve(work["inits"][:, :n_sample], work["params"][:, :n_sample], threads=threads)
+    res = run.solve(work["inits"][:, :n_sample], work["params"][:, :n_sample], threads=threads,
+                    **run_kw)
     return res, time.perf_counter() - t
 
 
 def cpu_baseline(work, kernel, seconds_target=15.0):
-    """Oracle on a bounded sample of the same workload, all host threads."""
+    """Oracle (the reference's algorithm restated in C++, FP64) on a bounded
+    sample of the same workload, all host threads."""
     threads = os.cpu_count() or 1
     n_total = work["inits"].shape[1]
     sample = min(n_total, 16 * threads)
@@ -218,8 +261,36 @@ VARIANTS = {
     "fp64_plain": (np.float64, "plain", "plain"),
     "fp32_plain": (np.float32, "plain", "plain"),
     "fp32_compensated_persistent": (np.float32, "persistent", "compensated"),
+    "fp64_persistent": (np.float64, "persistent", "plain"),
 }
-DEFAULT_VARIANTS = {"hh": ["fp64_plain"], "lorenz": ["fp64_plain"]}
+DEFAULT_VARIANTS = {"hh": ["fp32_compensated_persistent", "fp64_plain"], "lorenz": ["fp64_plain"],
+                    "vdp": ["fp32_compensated_persistent", "fp64_persistent", "fp64_plain"]}
+
+
+def tune_space(kernel_mode):
+    """Launch shapes the autotuner times: block size 256, occupancy floor of
+    the compiler's choice or 2 waves per SIMD, and for the persistent kernel
+    1, 2 or 4 workgroups per CU."""
+    return {"block_sizes": (256,), "waves_per_eu": (0, 2),
+            "blocks_per_cu": (1, 2, 4) if kernel_mode == "persistent" else (4,)}
+
+
+def tune(work, solver, sample_runs):
+    import torch
+    from cubie_amd.batchsolving.autotune import autotune
+    kernel = solver.kernel
+    prec = kernel.precision
+    m = min(sample_runs, work["inits"].shape[1])
+    inits = torch.from_numpy(np.ascontiguousarray(work["inits"][:, :m]).astype(prec)).cuda()
+    params = torch.from_numpy(np.ascontiguousarray(work["params"][:, :m]).astype(prec)).cuda()
+    run_kw = {k: np.asarray(v)[:m] for k, v in work.get("run_kw", {}).items()}
+    best = autotune(kernel, inits, params, work["duration"], sample_runs=m,
+                    kernel_mode=kernel.kernel_mode, **run_kw, **tune_space(kernel.kernel_mode))
+    for row in best["table"]:
+        print(f"# tune {kernel.precision.__name__} {kernel.kernel_mode}: {row}", file=sys.stderr,
+              flush=True)
+    return {k: best[k] for k in ("block_size", "waves_per_eu", "blocks_per_cu", "ms",
+                                 "sample_runs")}
 
 
 def make_solver(work, precision, kernel_mode, precision_mode):
@@ -230,6 +301,11 @@ def make_solver(work, precision, kernel_mode, precision_mode):
                        **w["solver_kw"])
     if w.get("drivers") is not None:
         solver._configure_drivers(w["drivers"])
+        interp = solver.kernel.driver_interpolator
+        if w.get("f32_problem") and interp.coefficients.dtype == np.float64:
+            # the FP32 kernels read FP32 driver coefficients: give the FP64
+            # kernels the same (FP32-representable) polynomial
+            interp.coefficients = interp.coefficients.astype(np.float32).astype(np.float64)
     return solver
 
 
@@ -239,13 +315,14 @@ def measure(work, solver, steps, warmup):
     prec = kernel.precision
     inits = torch.from_numpy(work["inits"].astype(prec)).cuda()
     params = torch.from_numpy(work["params"].astype(prec)).cuda()
+    run_kw = work.get("run_kw", {})
     for _ in range(warmup):
-        out, ms, info = kernel.run(inits, params, duration=work["duration"], timed=True)
+        out, ms, info = kernel.run(inits, params, duration=work["duration"], timed=True, **run_kw)
     torch.cuda.synchronize()
     times = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        out, ms, info = kernel.run(inits, params, duration=work["duration"], timed=True)
+        out, ms, info = kernel.run(inits, params, duration=work["duration"], timed=True, **run_kw)
         times.append(ms)
         print(f"# {kernel.precision.__name__} {kernel.kernel_mode}/{kernel.precision_mode}: "
               f"{ms:.1f} ms", file=sys.stderr, flush=True)
@@ -255,11 +332,16 @@ def measure(work, solver, steps, warmup):
     steps_total = int(ic[:, 2, :].astype(np.int64).sum())
     accepted = int((ic[:, 2, :].astype(np.int64) - ic[:, 3, :]).sum())
     status = out.status_codes.cpu().numpy()
+    bad = np.flatnonzero(status != 0)
+    if bad.size:
+        print(f"# failed runs ({bad.size}): " + ", ".join(
+            f"{int(i)}:{int(status[i])}" for i in bad[:40]), file=sys.stderr, flush=True)
     summ = out.state_summaries.cpu().numpy() if out.state_summaries.numel() > 1 else None
+    final = out.state[-1].cpu().numpy() if out.state.numel() > 1 else None
     mean_ms = float(np.mean(times))
     return {"ms": mean_ms, "wall_ms": wall / steps * 1e3, "steps": steps_total,
             "accepted": accepted, "value": accepted / (mean_ms * 1e-3),
-            "failed_runs": int((status != 0).sum()), "summaries": summ,
+            "failed_runs": int(bad.size), "status": status, "summaries": summ, "final": final,
             "spec": info["spec"], "kernel_info": info["module"].info()}
 
 
@@ -274,6 +356,9 @@ def main():
     ap.add_argument("--variants", default=None,
                     help="comma-separated subset of " + ",".join(VARIANTS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="launch with the default shape instead of autotuning it")
+    ap.add_argument("--tune-sample", type=int, default=262144)
     ap.add_argument("--compile-only", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead of the benchmark, run a plain device copy of this many "
@@ -301,9 +386,19 @@ def main():
     solvers = {nm: make_solver(work, *VARIANTS[nm]) for nm in names}
     if args.compile_only:
         for sv in solvers.values():
-            sv.kernel.compile_only(work["duration"])
+            k = sv.kernel
+            space = tune_space(k.kernel_mode)
+            shapes = ([(256, 0)] if args.no_tune else
+                      [(b, w) for b in space["block_sizes"] for w in space["waves_per_eu"]])
+            for b, w in shapes:
+                k.block_size, k.waves_per_eu = b, w
+                k.compile_only(work["duration"], "rtol_per_run" in work.get("run_kw", {}))
         return
 
+    launch = {}
+    if not args.no_tune:
+        for nm, sv in solvers.items():
+            launch[nm] = tune(work, sv, args.tune_sample)
     results = {nm: measure(work, sv, args.steps, args.warmup) for nm, sv in solvers.items()}
     # headline: the first listed variant whose runs all succeeded
     head = next((nm for nm in names if results[nm]["failed_runs"] == 0), names[0])
@@ -316,16 +411,43 @@ def main():
     variants_out = {}
     for nm, res in results.items():
         entry = {k: res[k] for k in ("value", "ms", "accepted", "failed_runs", "kernel_info")}
+        if nm in launch:
+            entry["launch"] = launch[nm]
         if ref is not None and res["summaries"] is not None and nm != "fp64_plain":
-            a, b = res["summaries"].astype(np.float64), ref
-            entry["max_rel_err_vs_fp64"] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+            # per summary column, over the systems both kernels finished
+            ok = (res["status"] == 0) & (results["fp64_plain"]["status"] == 0)
+            a, b = res["summaries"][..., ok].astype(np.float64), ref[..., ok]
+            scale = np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-300)
+            rel = (np.abs(a - b) / scale).max(axis=(0, 2)) if ok.any() else None
+            names = work.get("summary_columns") or [str(i) for i in range(a.shape[1])]
+            if rel is not None:
+                entry["rel_err_vs_fp64"] = {nm_: float(e) for nm_, e in zip(names, rel)}
+                entry["max_rel_err_vs_fp64"] = float(rel.max())
+                # integer-valued summaries: how many systems disagree at all
+                entry["systems_differing"] = {nm_: int((a[:, i] != b[:, i]).any(axis=0).sum())
+                                              for i, nm_ in enumerate(names) if "count" in nm_}
+            entry["systems_compared"] = int(ok.sum())
+        elif "fp64_plain" in results and res["final"] is not None and nm != "fp64_plain":
+            # final state, normwise relative difference per system
+            ok = (res["status"] == 0) & (results["fp64_plain"]["status"] == 0)
+            a = res["final"][..., ok].astype(np.float64)
+            b = results["fp64_plain"]["final"][..., ok]
+            e = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-300)
+            if e.size:
+                entry["final_state_rel_err_vs_fp64"] = {
+                    "median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)),
+                    "max": float(e.max())}
+            entry["systems_compared"] = int(ok.sum())
         if "fp64_plain" in results:
             entry["speedup_vs_fp64_plain"] = res["value"] / results["fp64_plain"]["value"]
+            entry["time_speedup_vs_fp64_plain"] = results["fp64_plain"]["ms"] / res["ms"]
         variants_out[nm] = entry
 
     cpu, orc_res, orc_steps = (None, None, 0)
     if not args.no_cpu:
-        cpu, orc_res, orc_steps = cpu_baseline(work, kernel)
+        base = (solvers["fp64_plain"] if "fp64_plain" in solvers
+                else make_solver(work, *VARIANTS["fp64_plain"]))
+        cpu, orc_res, orc_steps = cpu_baseline(work, base.kernel)
     if kernel.algorithm.family == "erk":
         fps = erk_flops_per_attempted_step(kernel, r["spec"])
     elif orc_res is not None:

@@ -66,7 +66,8 @@ class BatchSolverKernel:
                  algorithm_settings=None, output_settings=None, loop_settings=None,
                  kernel_mode: str = "plain", precision_mode: str = "plain",
                  fp_contract: Optional[bool] = None, block_size: int = 256,
-                 blocks_per_cu: int = 4, deterministic_pow: bool = False):
+                 blocks_per_cu: int = 4, deterministic_pow: bool = False,
+                 waves_per_eu: int = 0):
         self.system = system
         self.precision = system.precision
         if kernel_mode not in KERNEL_MODES:
@@ -79,7 +80,15 @@ class BatchSolverKernel:
         self.precision_mode = precision_mode
         self.block_size = int(block_size)
         self.blocks_per_cu = int(blocks_per_cu)
-        self.fp_contract = bool(fp_contract) if fp_contract is not None else False
+        # minimum resident waves per SIMD the compiler must allow (0: its own
+        # choice); a register budget of 512 / waves_per_eu VGPRs per lane
+        self.waves_per_eu = int(waves_per_eu)
+        # FMA contraction is on unless asked otherwise; the bit-for-bit parity
+        # mode (deterministic_pow) turns it off by default so the device
+        # rounds in the same order as the CPU oracle.
+        self._fp_contract_explicit = fp_contract is not None
+        self.fp_contract = (bool(fp_contract) if fp_contract is not None
+                            else not bool(deterministic_pow))
         self.deterministic_pow = bool(deterministic_pow)
         self.driver_interpolator = None
         self._algorithm_arg = algorithm
@@ -189,8 +198,14 @@ class BatchSolverKernel:
                 self._output_settings[k] = v
                 rec.add(k)
             elif k in ("kernel_mode", "precision_mode", "block_size", "blocks_per_cu", "fp_contract",
-                       "deterministic_pow"):
+                       "deterministic_pow", "waves_per_eu"):
                 setattr(self, k, v)
+                if k == "fp_contract":
+                    self._fp_contract_explicit = v is not None
+                    if v is None:
+                        self.fp_contract = not bool(self.deterministic_pow)
+                if k == "deterministic_pow" and not self._fp_contract_explicit:
+                    self.fp_contract = not bool(v)
                 rec.add(k)
         if rec:
             self._configure()
@@ -218,10 +233,12 @@ class BatchSolverKernel:
                                  f"({duration}), so this loop will produce no summary outputs")
 
     # -- spec / module --------------------------------------------------------
-    def spec_for(self, duration: float, per_run_rtol: bool = False) -> KernelSpec:
+    def spec_for(self, duration: float, per_run_rtol: bool = False, adjoint: bool = False,
+                 adj_ckpt_every: int = 32) -> KernelSpec:
         timing = self.timing.with_duration(duration)
         key = (timing.summarise_every, timing.sample_summaries_every, per_run_rtol,
-               self.precision_mode, self.fp_contract, self.block_size, self.deterministic_pow)
+               self.precision_mode, self.fp_contract, self.block_size, self.deterministic_pow,
+               self.waves_per_eu, bool(adjoint), int(adj_ckpt_every))
         if key not in self._spec_cache:
             self._spec_cache[key] = KernelSpec(
                 system=self.system, precision=self.precision, algorithm=self.algorithm,
@@ -229,12 +246,28 @@ class BatchSolverKernel:
                 drivers=self.driver_interpolator, implicit=self.implicit,
                 compensated=self.precision_mode == "compensated", per_run_rtol=per_run_rtol,
                 fp_contract=self.fp_contract, block_size=self.block_size,
-                deterministic_pow=self.deterministic_pow)
+                deterministic_pow=self.deterministic_pow, waves_per_eu=self.waves_per_eu,
+                adjoint=bool(adjoint), adj_ckpt_every=int(adj_ckpt_every))
         return self._spec_cache[key]
+
+    def config_key(self, duration: float, per_run_rtol: bool = False) -> str:
+        """Digest of everything that fixes the kernel's work except the launch
+        shape (block size, occupancy floor): two solvers with equal keys run
+        the same integration, so a launch shape tuned for one suits both."""
+        import hashlib
+        saved = (self.block_size, self.waves_per_eu)
+        self.block_size, self.waves_per_eu = 256, 0
+        try:
+            spec = self.spec_for(duration, per_run_rtol)
+            text = spec.source() + "\n" + "\n".join(spec.compile_options())
+        finally:
+            self.block_size, self.waves_per_eu = saved
+        return hashlib.sha256(text.encode()).hexdigest()
 
     def module(self, spec: KernelSpec, kernel_mode: Optional[str] = None):
         mode = kernel_mode or self.kernel_mode
-        name = "cubie_integrate" if mode == "plain" else "cubie_integrate_persistent"
+        name = {"plain": "cubie_integrate", "persistent": "cubie_integrate_persistent",
+                "record": "cubie_integrate_record", "adjoint": "cubie_adjoint"}[mode]
         return kcache.get_module(spec.source(), spec.compile_options(), name)
 
     def compile_only(self, duration: float = 1.0, per_run_rtol: bool = False) -> bytes:
@@ -269,6 +302,34 @@ class BatchSolverKernel:
             spec = self.spec_for(dur, rtol_per_run is not None)
             kcache.get_code(spec.source(), spec.compile_options())
             raise CompileOnly(spec)
+        st = self.setup(inits, params, duration, warmup, t0, rtol_per_run, duration_per_run)
+        spec = self.spec_for(st["duration"], st["per_run_rtol"])
+        mode = kernel_mode or self.kernel_mode
+        mod = self.module(spec, mode)
+        torch = _torch()
+        n_runs, args = st["n_runs"], st["args"]
+        block = int(blocksize or self.block_size)
+        if block != spec.block_size:
+            block = spec.block_size
+        if mode == "plain":
+            grid = max(1, math.ceil(n_runs / block))
+        else:
+            cus = torch.cuda.get_device_properties(torch.device("cuda")).multi_processor_count
+            grid = max(1, min(math.ceil(n_runs / block), cus * self.blocks_per_cu))
+        stream = torch.cuda.current_stream().cuda_stream
+        ms = None
+        if timed:
+            ms = mod.launch_timed(args, grid, block, stream)
+        else:
+            mod.launch(args, grid, block, stream)
+        info = {"grid": grid, "block": block, "mode": mode, "shapes": st["shapes"], "spec": spec,
+                "module": mod, "args": args, "keepalive": st["keepalive"]}
+        return st["outputs"], ms, info
+
+    def setup(self, inits, params, duration: float, warmup: float = 0.0, t0: float = 0.0,
+              rtol_per_run=None, duration_per_run=None) -> dict:
+        """Validate the run, place inputs on the device, allocate the outputs
+        and fill the kernel's argument block (``native.SolveArgs``)."""
         torch = _torch()
         prec = self.precision
         tdt = torch.float32 if prec is np.float32 else torch.float64
@@ -289,9 +350,6 @@ class BatchSolverKernel:
         timing = self.timing.with_duration(duration)
         self._validate_timing(timing, duration)
         per_run_rtol = rtol_per_run is not None
-        spec = self.spec_for(duration, per_run_rtol)
-        mode = kernel_mode or self.kernel_mode
-        mod = self.module(spec, mode)
 
         # run times cast to the working precision before the launch
         dur_p = float(prec(duration))
@@ -352,23 +410,9 @@ class BatchSolverKernel:
         args.summary_rows = max(shapes["state_summaries"][0], shapes["observable_summaries"][0])
         args.chunk_runs = 1
         args.reserved = 0
-
-        block = int(blocksize or self.block_size)
-        if block != spec.block_size:
-            block = spec.block_size
-        if mode == "plain":
-            grid = max(1, math.ceil(n_runs / block))
-        else:
-            cus = torch.cuda.get_device_properties(dev).multi_processor_count
-            grid = max(1, min(math.ceil(n_runs / block), cus * self.blocks_per_cu))
-        stream = torch.cuda.current_stream().cuda_stream
-        ms = None
-        if timed:
-            ms = mod.launch_timed(args, grid, block, stream)
-        else:
-            mod.launch(args, grid, block, stream)
         dout = DeviceOutputs(outs["state"], outs["observables"], outs["state_summaries"],
                              outs["observable_summaries"], outs["iteration_counters"], status)
-        info = {"grid": grid, "block": block, "mode": mode, "shapes": shapes, "spec": spec,
-                "module": mod, "args": args, "keepalive": (d_inits, d_params, coeffs, rtol_d, dur_d, counter)}
-        return dout, ms, info
+        return {"args": args, "outputs": dout, "shapes": shapes, "n_runs": n_runs,
+                "duration": duration, "per_run_rtol": per_run_rtol, "params": d_params,
+                "inits": d_inits,
+                "keepalive": (d_inits, d_params, coeffs, rtol_d, dur_d, counter)}

@@ -14,7 +14,8 @@ equations when given strings or a callable.
 Engine-specific keywords (not in the reference): ``kernel_mode``
 ("plain" | "persistent"), ``precision_mode`` ("plain" | "compensated"),
 ``fp_contract``, ``deterministic_pow`` (controller powers computed bit for
-bit like the CPU oracle), ``block_size``, ``blocks_per_cu`` and, at solve time,
+bit like the CPU oracle), ``block_size``, ``blocks_per_cu``, ``waves_per_eu``
+(occupancy floor the register allocator must honour) and, at solve time,
 ``rtol_per_run`` / ``duration_per_run`` arrays.
 """
 
@@ -43,7 +44,7 @@ ALL_MEMORY_MANAGER_PARAMETERS = {"memory_manager", "stream_group", "mem_proporti
                                  "host_spill_threshold", "spill_directory"}
 ALL_CACHE_PARAMETERS = {"cache", "cache_dir", "cache_mode", "max_cache_entries"}
 ENGINE_PARAMETERS = {"kernel_mode", "precision_mode", "fp_contract", "block_size", "blocks_per_cu",
-                     "deterministic_pow"}
+                     "deterministic_pow", "waves_per_eu"}
 
 
 def _check_renamed(keys: Iterable[str]) -> None:

@@ -175,13 +175,15 @@ int cubie_hip_kernel_info(cubie_module_t module, int* num_regs, int* shared_byte
     return CUBIE_OK;
 }
 
-static int launch_impl(cubie_module_t module, const cubie_solve_args* args,
-                       unsigned grid, unsigned block, unsigned dynamic_lds,
-                       hipStream_t stream) {
+// The kernels take their argument block by value: one kernel parameter.
+extern "C++" {
+template <typename Args>
+static int launch_impl(cubie_module_t module, const Args* args, unsigned grid, unsigned block,
+                       unsigned dynamic_lds, hipStream_t stream) {
     if (!module || !args) return fail(CUBIE_E_INVALID, "null argument");
     if (grid == 0 || block == 0 || block > 1024)
         return fail(CUBIE_E_INVALID, "invalid launch shape");
-    cubie_solve_args local = *args;
+    Args local = *args;
     void* params[] = {&local};
     hipError_t err = hipModuleLaunchKernel(module->function, grid, 1, 1, block, 1, 1,
                                            dynamic_lds, stream, params, nullptr);
@@ -189,16 +191,11 @@ static int launch_impl(cubie_module_t module, const cubie_solve_args* args,
     return CUBIE_OK;
 }
 
-int cubie_hip_launch(cubie_module_t module, const cubie_solve_args* args,
-                     unsigned grid, unsigned block, unsigned dynamic_lds, void* stream) {
-    return launch_impl(module, args, grid, block, dynamic_lds,
-                       reinterpret_cast<hipStream_t>(stream));
-}
-
-int cubie_hip_launch_timed(cubie_module_t module, const cubie_solve_args* args,
-                           unsigned grid, unsigned block, unsigned dynamic_lds,
-                           void* stream, float* elapsed_ms) {
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+// Launch bracketed by a hipEvent pair on the stream; waits for the stop event.
+template <typename Args>
+static int launch_timed_impl(cubie_module_t module, const Args* args, unsigned grid,
+                             unsigned block, unsigned dynamic_lds, hipStream_t s,
+                             float* elapsed_ms) {
     hipEvent_t start, stop;
     hipError_t err = hipEventCreate(&start);
     if (err != hipSuccess) return hip_fail(err, "hipEventCreate");
@@ -226,6 +223,33 @@ int cubie_hip_launch_timed(cubie_module_t module, const cubie_solve_args* args,
     hipEventDestroy(start);
     hipEventDestroy(stop);
     return rc;
+}
+}  // extern "C++"
+
+int cubie_hip_launch(cubie_module_t module, const cubie_solve_args* args,
+                     unsigned grid, unsigned block, unsigned dynamic_lds, void* stream) {
+    return launch_impl(module, args, grid, block, dynamic_lds,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+int cubie_hip_launch_timed(cubie_module_t module, const cubie_solve_args* args,
+                           unsigned grid, unsigned block, unsigned dynamic_lds,
+                           void* stream, float* elapsed_ms) {
+    return launch_timed_impl(module, args, grid, block, dynamic_lds,
+                             reinterpret_cast<hipStream_t>(stream), elapsed_ms);
+}
+
+int cubie_hip_launch_adjoint(cubie_module_t module, const cubie_adjoint_args* args,
+                             unsigned grid, unsigned block, unsigned dynamic_lds, void* stream) {
+    return launch_impl(module, args, grid, block, dynamic_lds,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+int cubie_hip_launch_adjoint_timed(cubie_module_t module, const cubie_adjoint_args* args,
+                                   unsigned grid, unsigned block, unsigned dynamic_lds,
+                                   void* stream, float* elapsed_ms) {
+    return launch_timed_impl(module, args, grid, block, dynamic_lds,
+                             reinterpret_cast<hipStream_t>(stream), elapsed_ms);
 }
 
 }  // extern "C"

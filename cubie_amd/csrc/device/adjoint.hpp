@@ -1,0 +1,531 @@
+// Discrete adjoint of the integration loop: gradients of a loss on the saved
+// states and state summaries with respect to the initial values and the
+// parameters of every system.
+//
+// The adjoint is that of the discrete map the forward pass computed, with
+// its accepted step sizes held fixed (the controller's dependence of h on
+// the state is not differentiated, as usual for discrete adjoints of
+// adaptive integrators). The recording forward pass (cubie_integrate_record,
+// Tape in loop.hpp) stores per system the size of every accepted step, one
+// byte of output flags per step, the state and time every
+// Cfg::adj_ckpt_every accepted steps, and the step / save / sample totals.
+//
+// The backward pass (cubie_adjoint) walks the checkpoint segments last to
+// first. For a segment it replays the taped steps from the checkpoint,
+// keeping each step's start state and stage values in a per-system replay
+// buffer in HBM, then runs the step adjoints last to first, adding the
+// output gradients of each step's end state before propagating through the
+// step. Replay buffers: [K+1][n][N] states, [K+1][N] times, [K][stash][N]
+// stage values, with the run axis fastest (coalesced across a wave).
+//
+// One step's adjoint, lambda = dL/dy_{k+1} in, dL/dy_k out:
+//   explicit RK: y+ = y + h sum_i b_i k_i, k_i = f(Y_i), Y_i = y + h sum_j a_ij k_j
+//     g_i = h b_i lambda + h sum_{j>i} a_ji w_j,  w_i = J(Y_i)^T g_i,
+//     dL/dy_k = lambda + sum_i w_i,  dL/dp += sum_i f_p(Y_i)^T g_i.
+//   Rosenbrock-W: W = I - gamma h J(y_n), f_t at (t_n, y_n),
+//     r_i = gamma (h f(Y_i) + sum_{j<i} C_ij K_j + gamma_i h^2 f_t),
+//     K_i = W^-1 r_i, Y_i = y_n + sum_{j<i} a_ij K_j, y+ = y_n + sum_i b_i K_i.
+//     With Kbar_i = dL/dK_i (b_i lambda plus later stages), z_i = W^-T Kbar_i:
+//       Ybar_i = gamma h J(Y_i)^T z_i, Kbar_j += gamma C_ij z_i + a_ij Ybar_i,
+//       ybar += Ybar_i + gamma h grad_y(z_i^T J(y_n) K_i)
+//               + gamma gamma_i h^2 grad_y(z_i^T f_t(y_n)),
+//       pbar likewise with grad_p and f_p.
+//     The stage solves are exact (dense LU); the forward Krylov solves agree
+//     with them to the Krylov tolerance.
+// The reference has no adjoint; nothing here has a counterpart there.
+#pragma once
+
+namespace cubie {
+
+// ---------------------------------------------------------------------------
+// Small dense LU with partial pivoting (row swaps applied as predicated
+// selects, so every index stays a compile-time constant and the factors live
+// in registers).
+// ---------------------------------------------------------------------------
+template <typename T, int n>
+struct DenseLU {
+    T a[n * n];
+    int piv[n];
+
+    __device__ __forceinline__ void factor(const T* M) {
+#pragma unroll
+        for (int i = 0; i < n * n; ++i) a[i] = M[i];
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            int p = k;
+            T best = dabs(a[k * n + k]);
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const T v = dabs(a[i * n + k]);
+                const bool better = v > best;
+                best = better ? v : best;
+                p = better ? i : p;
+            }
+            piv[k] = p;
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const bool sw = (i == p);
+#pragma unroll
+                for (int j = 0; j < n; ++j) {
+                    const T x = a[k * n + j], y = a[i * n + j];
+                    a[k * n + j] = sw ? y : x;
+                    a[i * n + j] = sw ? x : y;
+                }
+            }
+            const T d = a[k * n + k];
+            const T inv = d != T(0) ? T(1) / d : T(0);
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const T l = a[i * n + k] * inv;
+                a[i * n + k] = l;
+#pragma unroll
+                for (int j = k + 1; j < n; ++j) a[i * n + j] -= l * a[k * n + j];
+            }
+        }
+    }
+
+    // M x = b, in place
+    __device__ __forceinline__ void solve(T* x) const {
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const bool sw = (i == piv[k]);
+                const T u = x[k], v = x[i];
+                x[k] = sw ? v : u;
+                x[i] = sw ? u : v;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+#pragma unroll
+            for (int j = 0; j < i; ++j) x[i] -= a[i * n + j] * x[j];
+#pragma unroll
+        for (int i = n - 1; i >= 0; --i) {
+#pragma unroll
+            for (int j = i + 1; j < n; ++j) x[i] -= a[i * n + j] * x[j];
+            x[i] = x[i] / a[i * n + i];
+        }
+    }
+
+    // M^T x = b, in place (M = P^T L U: U^T, then L^T, then the swaps reversed)
+    __device__ __forceinline__ void solve_t(T* x) const {
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+#pragma unroll
+            for (int j = 0; j < i; ++j) x[i] -= a[j * n + i] * x[j];
+            x[i] = x[i] / a[i * n + i];
+        }
+#pragma unroll
+        for (int i = n - 1; i >= 0; --i)
+#pragma unroll
+            for (int j = i + 1; j < n; ++j) x[i] -= a[j * n + i] * x[j];
+#pragma unroll
+        for (int k = n - 1; k >= 0; --k) {
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const bool sw = (i == piv[k]);
+                const T u = x[k], v = x[i];
+                x[k] = sw ? v : u;
+                x[i] = sw ? u : v;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Explicit Runge-Kutta step adjoint
+// ---------------------------------------------------------------------------
+template <typename Cfg, typename Sys>
+struct ErkAdjoint {
+    using real = typename Cfg::real;
+    static constexpr int n = Cfg::n;
+    static constexpr int s = Cfg::stages;
+    static constexpr int P = Cfg::n_params;
+    static constexpr int stash = s * n;   // the stage derivatives k_i
+
+    __device__ __forceinline__ static void stage_state(const real* y, real h, const real* kst, int i,
+                                                       real* Y) {
+#pragma unroll
+        for (int c = 0; c < n; ++c) {
+            real acc = real(0);
+#pragma unroll
+            for (int j = 0; j < s; ++j)
+                if (j < i && Cfg::A[i][j] != 0.0) acc += Cfg::A[i][j] * kst[j * n + c];
+            Y[c] = acc * h + y[c];
+        }
+    }
+
+    // Replay one step from y at (t, h): y <- y+, kst <- k_1..k_s. The stage
+    // sums use the forward step's order of operations.
+    template <typename Ctx>
+    __device__ __forceinline__ static void replay(Ctx& ctx, real* y, real t, real h, real* kst) {
+#pragma unroll
+        for (int i = 0; i < s; ++i) {
+            real Y[n];
+            stage_state(y, h, kst, i, Y);
+            const real ts = t + h * Cfg::c[i];
+            ctx.drv.eval(ts, ctx.proposed_drivers.data());
+            Sys::dxdt(Y, ctx.params.data(), ctx.proposed_drivers.data(),
+                      ctx.proposed_observables.data(), &kst[i * n], ts);
+        }
+#pragma unroll
+        for (int c = 0; c < n; ++c) {
+            real acc = real(0);
+#pragma unroll
+            for (int i = 0; i < s; ++i)
+                if (Cfg::b[i] != 0.0) acc += Cfg::b[i] * kst[i * n + c];
+            y[c] = acc * h + y[c];
+        }
+    }
+
+    // lam: dL/dy_{k+1} in, dL/dy_k out; mu accumulates dL/dp.
+    template <typename Ctx>
+    __device__ __forceinline__ static void backward(Ctx& ctx, const real* y, real t, real h,
+                                                    const real* kst, real* lam, real* mu) {
+        real G[s * n];
+#pragma unroll
+        for (int i = 0; i < s; ++i)
+#pragma unroll
+            for (int c = 0; c < n; ++c) G[i * n + c] = (h * Cfg::b[i]) * lam[c];
+#pragma unroll
+        for (int i = s - 1; i >= 0; --i) {
+            // stage i matters if it feeds the solution or a later stage
+            bool live = Cfg::b[i] != 0.0;
+#pragma unroll
+            for (int j = 0; j < s; ++j) live = live || (j > i && Cfg::A[j][i] != 0.0);
+            if (!live) continue;
+            real Y[n];
+            stage_state(y, h, kst, i, Y);
+            const real ts = t + h * Cfg::c[i];
+            ctx.drv.eval(ts, ctx.proposed_drivers.data());
+            real w[n], wp[P > 0 ? P : 1];
+            Sys::vjp(Y, ctx.params.data(), ctx.proposed_drivers.data(), &G[i * n], w, wp, ts);
+#pragma unroll
+            for (int c = 0; c < n; ++c) lam[c] += w[c];
+#pragma unroll
+            for (int q = 0; q < P; ++q) mu[q] += wp[q];
+#pragma unroll
+            for (int j = 0; j < s; ++j) {
+                if (j < i && Cfg::A[i][j] != 0.0) {
+#pragma unroll
+                    for (int c = 0; c < n; ++c) G[j * n + c] += (h * Cfg::A[i][j]) * w[c];
+                }
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Rosenbrock-W step adjoint (dense stage solves; small systems)
+// ---------------------------------------------------------------------------
+template <typename Cfg, typename Sys>
+struct RosenbrockAdjoint {
+    using real = typename Cfg::real;
+    static constexpr int n = Cfg::n;
+    static constexpr int s = Cfg::stages;
+    static constexpr int P = Cfg::n_params;
+    static constexpr int nd = Cfg::n_drivers;
+    static constexpr int stash = s * n;   // the stage increments K_i
+
+    // W = I - gamma h J(t, y) and the step's h f_t(t, y), drivers at t.
+    template <typename Ctx>
+    __device__ __forceinline__ static void step_matrix(Ctx& ctx, const real* y, real t, real h,
+                                                       DenseLU<real, n>& lu, real* fdt,
+                                                       real* dd) {
+        ctx.drv.eval(t, ctx.drivers.data());
+        real J[n * n];
+        Sys::jac(y, ctx.params.data(), ctx.drivers.data(), J, t);
+        const real gh = real(Cfg::gamma * h);
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+#pragma unroll
+            for (int j = 0; j < n; ++j) J[i * n + j] = (i == j ? real(1) : real(0)) - gh * J[i * n + j];
+        lu.factor(J);
+#pragma unroll
+        for (int q = 0; q < (nd > 0 ? nd : 1); ++q) dd[q] = real(0);
+        if constexpr (nd > 0) ctx.drv.eval_dt(t, dd);
+        Sys::dfdt(y, ctx.params.data(), ctx.drivers.data(), dd, fdt, t);
+#pragma unroll
+        for (int i = 0; i < n; ++i) fdt[i] *= h;
+    }
+
+    __device__ __forceinline__ static void stage_state(const real* y, const real* kst, int i,
+                                                       real* Y) {
+#pragma unroll
+        for (int c = 0; c < n; ++c) {
+            real v = y[c];
+#pragma unroll
+            for (int j = 0; j < s; ++j)
+                if (j < i && Cfg::A[i][j] != 0.0) v += Cfg::A[i][j] * kst[j * n + c];
+            Y[c] = v;
+        }
+    }
+
+    template <typename Ctx>
+    __device__ __forceinline__ static void replay(Ctx& ctx, real* y, real t, real h, real* kst) {
+        DenseLU<real, n> lu;
+        real fdt[n], dd[nd > 0 ? nd : 1];
+        step_matrix(ctx, y, t, h, lu, fdt, dd);
+        const real inv_h = real(1) / h;
+#pragma unroll
+        for (int i = 0; i < s; ++i) {
+            real Y[n], f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1];
+            stage_state(y, kst, i, Y);
+            const real ts = t + h * Cfg::c[i];
+            ctx.drv.eval(ts, ctx.proposed_drivers.data());
+            Sys::dxdt(Y, ctx.params.data(), ctx.proposed_drivers.data(), obs, f, ts);
+#pragma unroll
+            for (int c = 0; c < n; ++c) {
+                real corr = real(0);
+#pragma unroll
+                for (int j = 0; j < s; ++j)
+                    if (j < i && Cfg::rbC[i][j] != 0.0) corr += Cfg::rbC[i][j] * kst[j * n + c];
+                const real v = f[c] + corr * inv_h + Cfg::gamma_stages[i] * fdt[c];
+                kst[i * n + c] = v * h * Cfg::gamma;
+            }
+            lu.solve(&kst[i * n]);
+        }
+#pragma unroll
+        for (int c = 0; c < n; ++c) {
+            real v = y[c];
+#pragma unroll
+            for (int i = 0; i < s; ++i)
+                if (Cfg::b[i] != 0.0) v += Cfg::b[i] * kst[i * n + c];
+            y[c] = v;
+        }
+    }
+
+    template <typename Ctx>
+    __device__ __forceinline__ static void backward(Ctx& ctx, const real* y, real t, real h,
+                                                    const real* kst, real* lam, real* mu) {
+        DenseLU<real, n> lu;
+        real fdt[n], dd[nd > 0 ? nd : 1];
+        step_matrix(ctx, y, t, h, lu, fdt, dd);
+        real Kb[s * n];
+#pragma unroll
+        for (int i = 0; i < s; ++i)
+#pragma unroll
+            for (int c = 0; c < n; ++c) Kb[i * n + c] = Cfg::b[i] * lam[c];
+        const real gh = real(Cfg::gamma * h);
+#pragma unroll
+        for (int i = s - 1; i >= 0; --i) {
+            real z[n];
+#pragma unroll
+            for (int c = 0; c < n; ++c) z[c] = Kb[i * n + c];
+            lu.solve_t(z);
+            // through f(Y_i): weight gamma h z
+            real Y[n], wz[n], w[n], wp[P > 0 ? P : 1];
+            stage_state(y, kst, i, Y);
+            const real ts = t + h * Cfg::c[i];
+            ctx.drv.eval(ts, ctx.proposed_drivers.data());
+#pragma unroll
+            for (int c = 0; c < n; ++c) wz[c] = gh * z[c];
+            Sys::vjp(Y, ctx.params.data(), ctx.proposed_drivers.data(), wz, w, wp, ts);
+#pragma unroll
+            for (int c = 0; c < n; ++c) lam[c] += w[c];
+#pragma unroll
+            for (int q = 0; q < P; ++q) mu[q] += wp[q];
+#pragma unroll
+            for (int j = 0; j < s; ++j) {
+                if (j < i) {
+#pragma unroll
+                    for (int c = 0; c < n; ++c) {
+                        real add = real(0);
+                        if (Cfg::A[i][j] != 0.0) add += Cfg::A[i][j] * w[c];
+                        if (Cfg::rbC[i][j] != 0.0) add += real(Cfg::gamma * Cfg::rbC[i][j]) * z[c];
+                        Kb[j * n + c] += add;
+                    }
+                }
+            }
+            // through W = I - gamma h J(y_n): + gamma h grad(z^T J K_i)
+            real hy[n], hp[P > 0 ? P : 1];
+            Sys::hvp(y, ctx.params.data(), ctx.drivers.data(), &kst[i * n], wz, hy, hp, t);
+#pragma unroll
+            for (int c = 0; c < n; ++c) lam[c] += hy[c];
+#pragma unroll
+            for (int q = 0; q < P; ++q) mu[q] += hp[q];
+            // through the time-derivative term gamma gamma_i h^2 f_t(y_n)
+            if (Cfg::gamma_stages[i] != 0.0) {
+                real zt[n], ty[n], tp[P > 0 ? P : 1];
+#pragma unroll
+                for (int c = 0; c < n; ++c) zt[c] = real(Cfg::gamma * Cfg::gamma_stages[i]) * (h * h) * z[c];
+                Sys::ftvjp(y, ctx.params.data(), ctx.drivers.data(), dd, zt, ty, tp, t);
+#pragma unroll
+                for (int c = 0; c < n; ++c) lam[c] += ty[c];
+#pragma unroll
+                for (int q = 0; q < P; ++q) mu[q] += tp[q];
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Forward pass that also records the tape (one lane per system).
+template <typename Cfg, typename Sys, typename Stepper>
+__device__ __forceinline__ void integrate_record(const cubie_adjoint_args& A) {
+    const cubie_solve_args& a = A.fwd;
+    const long long run = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (run >= a.n_runs) return;
+    Integrator<Cfg, Sys, Stepper, true> it;
+    it.tape.bind(A, run);
+    it.begin(a, run);
+    while (!it.advance(a)) {
+    }
+    const long long N = a.n_runs;
+    a.status_out[run] = it.status | (it.tape.full ? ST_TAPE_FULL : 0);
+    A.tape_info[run] = int(it.tape.steps);
+    A.tape_info[N + run] = int(it.save_idx);
+    A.tape_info[2 * N + run] = int(it.update_idx);
+}
+
+// Gradient contributions of the outputs taken at one step's end state.
+template <typename Cfg>
+__device__ __forceinline__ void add_output_grads(const cubie_adjoint_args& A, long long run,
+                                                 unsigned flags, long long& save_i, long long& upd_i,
+                                                 const typename Cfg::real* y_end,
+                                                 typename Cfg::real* lam) {
+    using real = typename Cfg::real;
+    using io_real = typename Cfg::io_real;
+    const cubie_solve_args& a = A.fwd;
+    const long long N = a.n_runs;
+    if (flags & 1u) {
+        save_i -= 1;
+        if constexpr (Cfg::save_state) {
+            constexpr int W = Cfg::n_saved_states + (Cfg::save_time ? 1 : 0);
+            const io_real* g = static_cast<const io_real*>(A.grad_state);
+            if (g != nullptr && save_i < a.out_rows) {
+#pragma unroll
+                for (int k = 0; k < Cfg::n_saved_states; ++k)
+                    lam[Cfg::saved_state_idx[k]] += real(g[(save_i * W + k) * N + run]);
+            }
+        }
+    }
+    if (flags & 2u) {
+        upd_i -= 1;
+        if constexpr (Cfg::summarise_state) {
+            constexpr int NS = Cfg::n_summ_states;
+            constexpr int OH = Cfg::summ_out_height;
+            const long long row = upd_i / Cfg::samples_per_summary;
+            const io_real* g = static_cast<const io_real*>(A.grad_state_summ);
+            const io_real* out = static_cast<const io_real*>(a.state_summ_out);
+            if (g != nullptr && row < a.summary_rows) {
+                const real inv_spp = real(1.0 / double(Cfg::samples_per_summary));
+#pragma unroll
+                for (int v = 0; v < NS; ++v) {
+                    const int si = Cfg::summ_state_idx[v];
+                    const real x = y_end[si];
+                    int ooff = 0;
+#pragma unroll
+                    for (int m = 0; m < Cfg::n_metrics; ++m) {
+                        const long long col = (row * (NS * OH) + v * OH + ooff) * N + run;
+                        const int id = Cfg::metric_ids[m];
+                        if (id == M_MEAN) {
+                            lam[si] += real(g[col]) * inv_spp;
+                        } else if (id == M_RMS) {
+                            const real r = real(out[col]);
+                            lam[si] += r != real(0) ? real(g[col]) * x * inv_spp / r : real(0);
+                        } else if (id == M_MAX || id == M_MIN) {
+                            lam[si] += (io_real(x) == out[col]) ? real(g[col]) : real(0);
+                        }
+                        ooff += Cfg::metric_out_sizes[m];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Backward pass: gradients of the loss with respect to inits and params.
+template <typename Cfg, typename Sys, typename Adj>
+__device__ __forceinline__ void adjoint_backward(const cubie_adjoint_args& A) {
+    using real = typename Cfg::real;
+    using io_real = typename Cfg::io_real;
+    constexpr int n = Cfg::n;
+    constexpr int P = Cfg::n_params;
+    constexpr int S = Adj::stash;
+    constexpr int K = Cfg::adj_ckpt_every;
+    const cubie_solve_args& a = A.fwd;
+    const long long run = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (run >= a.n_runs) return;
+    const long long N = a.n_runs;
+
+    LoopCtx<Cfg> ctx;
+    ctx.drv.coeffs = static_cast<const io_real*>(a.driver_coeffs);
+    const io_real* params = static_cast<const io_real*>(a.params);
+#pragma unroll
+    for (int q = 0; q < P; ++q) ctx.params[q] = real(params[q * N + run]);
+
+    real lam[n], mu[P > 0 ? P : 1];
+#pragma unroll
+    for (int c = 0; c < n; ++c) lam[c] = real(0);
+#pragma unroll
+    for (int q = 0; q < (P > 0 ? P : 1); ++q) mu[q] = real(0);
+
+    const long long steps = A.tape_info[run];
+    long long save_i = A.tape_info[N + run];
+    long long upd_i = A.tape_info[2 * N + run];
+    const real* tdt = static_cast<const real*>(A.tape_dt);
+    const real* ck = static_cast<const real*>(A.checkpoints);
+    real* rep_y = static_cast<real*>(A.replay);
+    real* rep_k = static_cast<real*>(A.replay_stages);
+    double* rep_t = A.replay_t;
+    const long long nseg = (steps + K - 1) / K;
+
+    for (long long seg = nseg - 1; seg >= 0; --seg) {
+        const long long k0 = seg * K;
+        const int m = int((steps - k0) < K ? (steps - k0) : K);
+        real y[n];
+#pragma unroll
+        for (int c = 0; c < n; ++c) y[c] = ck[(seg * n + c) * N + run];
+        double t = A.checkpoint_t[seg * N + run];
+        for (int j = 0; j < m; ++j) {
+#pragma unroll
+            for (int c = 0; c < n; ++c) rep_y[((long long)j * n + c) * N + run] = y[c];
+            rep_t[(long long)j * N + run] = t;
+            const real h = tdt[(k0 + j) * N + run];
+            real kst[S];
+            Adj::replay(ctx, y, real(t), h, kst);
+#pragma unroll
+            for (int q = 0; q < S; ++q) rep_k[((long long)j * S + q) * N + run] = kst[q];
+            t += double(h);
+        }
+#pragma unroll
+        for (int c = 0; c < n; ++c) rep_y[((long long)m * n + c) * N + run] = y[c];
+        for (int j = m - 1; j >= 0; --j) {
+            const unsigned flags = A.tape_flags[(k0 + j) * N + run];
+            if (flags) {
+                real y_end[n];
+#pragma unroll
+                for (int c = 0; c < n; ++c) y_end[c] = rep_y[((long long)(j + 1) * n + c) * N + run];
+                add_output_grads<Cfg>(A, run, flags, save_i, upd_i, y_end, lam);
+            }
+            real y0[n], kst[S];
+#pragma unroll
+            for (int c = 0; c < n; ++c) y0[c] = rep_y[((long long)j * n + c) * N + run];
+#pragma unroll
+            for (int q = 0; q < S; ++q) kst[q] = rep_k[((long long)j * S + q) * N + run];
+            const real tj = real(rep_t[(long long)j * N + run]);
+            const real h = tdt[(k0 + j) * N + run];
+            Adj::backward(ctx, y0, tj, h, kst, lam, mu);
+        }
+    }
+    // the save row written at t0 (no settling time)
+    if (a.warmup == 0.0 && save_i > 0) {
+        real y0[n];
+#pragma unroll
+        for (int c = 0; c < n; ++c) y0[c] = ck[c * N + run];
+        add_output_grads<Cfg>(A, run, 1u, save_i, upd_i, y0, lam);
+    }
+    io_real* gi = static_cast<io_real*>(A.grad_inits);
+    io_real* gp = static_cast<io_real*>(A.grad_params);
+#pragma unroll
+    for (int c = 0; c < n; ++c) gi[c * N + run] = io_real(lam[c]);
+#pragma unroll
+    for (int q = 0; q < P; ++q) gp[q * N + run] = io_real(mu[q]);
+}
+
+}  // namespace cubie

@@ -21,6 +21,8 @@ enum : int {
     ST_STAGNATION = 64,
     ST_BICGSTAB_BREAKDOWN = 128,
     ST_NEWTON_DIVERGENCE = 256,
+    // engine extension: the adjoint's recording buffers were too small
+    ST_TAPE_FULL = 4096,
 };
 
 // Python's max()/min() selection order (the reference's device code calls

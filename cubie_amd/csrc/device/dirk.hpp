@@ -180,11 +180,20 @@ struct DIRKStep {
 #pragma unroll
                 for (int i = 0; i < n; ++i) stage_base[i] += diag * stage_increment[i];
             }
-            if constexpr (Cfg::n_obs > 0)
-                Sys::observables(stage_base.data(), ctx.params.data(), ctx.proposed_drivers.data(),
-                                 ctx.proposed_observables.data(), stage_time);
-            Sys::dxdt(stage_base.data(), ctx.params.data(), ctx.proposed_drivers.data(),
-                      ctx.proposed_observables.data(), stage_rhs.data(), stage_time);
+            // The last stage's derivative feeds nothing when the solution is a
+            // copy of that stage (b = last row of A), no error sum or FSAL
+            // cache needs it and no prediction history reads it: skip it.
+            constexpr bool last_rhs_unused =
+                !Cfg::fsal && !Cfg::accumulates_output &&
+                !(Cfg::has_error && Cfg::accumulates_error) &&
+                !(Cfg::dense_prediction && Cfg::A[s - 1][s - 1] == 0.0);
+            if (!(last_rhs_unused && stage_idx == s - 1)) {
+                if constexpr (Cfg::n_obs > 0)
+                    Sys::observables(stage_base.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                                     ctx.proposed_observables.data(), stage_time);
+                Sys::dxdt(stage_base.data(), ctx.params.data(), ctx.proposed_drivers.data(),
+                          ctx.proposed_observables.data(), stage_rhs.data(), stage_time);
+            }
             if constexpr (Cfg::dense_prediction && Cfg::dirk_has_later_explicit) {
                 if (diag == real(0)) {
 #pragma unroll

@@ -58,6 +58,7 @@ struct FirkProblem {
 
     static constexpr bool cache = Cfg::cache_jac;
     Vec<L, (cache ? s * n * n : 1)> J;   // J(Y_i) per stage, refreshed per linear solve
+    Vec<L, n> kw;                        // 1 / Krylov tolerance weights (per state)
 
     // Called by the Newton iteration after each residual (fixed iterate
     // during the Krylov solve that follows).
@@ -69,6 +70,11 @@ struct FirkProblem {
                 stage_state_l(i, Y);
                 Sys::jac(Y, params_l, drivers_l + i * nd, J.data() + i * n * n, L(stage_time(i)));
             }
+        }
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            L tol = L(Cfg::krylov_atol[k]) + L(Cfg::krylov_rtol[k]) * dabs(L(base[k]));
+            kw[k] = L(1) / pmax(tol, L(1e-16));
         }
     }
     // J(Y_i) dir for stage i
@@ -140,10 +146,7 @@ struct FirkProblem {
         L nrm2 = L(0);
 #pragma unroll
         for (int e = 0; e < W; ++e) {
-            const int k = e % n;
-            L tol = L(Cfg::krylov_atol[k]) + L(Cfg::krylov_rtol[k]) * dabs(L(base[k]));
-            tol = pmax(tol, L(1e-16));
-            const L r = dabs(vals[e]) / tol;
+            const L r = dabs(vals[e]) * kw[e % n];
             nrm2 += r * r;
         }
         return nrm2 * L(1.0 / double(W));

@@ -324,6 +324,15 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         const real norm2_dz = P.correction_norm2(delta);
         const real ndz = real(dsqrt(norm2_dz));
         const bool judged = lin == ST_SUCCESS;
+        // FP32 Krylov solves of the mixed mode: a solve that stops short
+        // (iteration cap, or a BiCGSTAB breakdown where the operator's
+        // symmetric part is indefinite) still returns a correction that
+        // lowers the FP64 residual. It is applied, but it neither ends the
+        // iteration nor enters the contraction estimate (the next converged
+        // correction starts a fresh history).
+        const bool partial = Cfg::newton_partial && !judged &&
+                             (lin == ST_LINEAR_MAX_ITERS || lin == ST_BICGSTAB_BREAKDOWN);
+        const bool usable = judged || partial;
         const bool history = ndz_prev > real(0);
         const real ndz_prev_safe = history ? ndz_prev : real(1);
         const real theta = history ? pmax(theta_decay * prev_theta, ndz / ndz_prev_safe) : prev_theta;
@@ -331,15 +340,15 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         const bool eta_accept = (theta < real(1)) && (theta * ndz < kappa * (real(1) - theta));
         const bool nonfinite = !(norm2_dz <= huge);
         const bool stagnant = judged && history && (dabs(theta - real(1)) <= stag_eps);
-        const bool diverging = judged && ((history && (theta > theta_div)) || nonfinite);
+        const bool diverging = (judged && history && (theta > theta_div)) || (usable && nonfinite);
         const bool converged_stagnant = stagnant && (ndz <= real(1)) && !diverging;
         const bool failed_now = diverging || (stagnant && (ndz > real(1)));
         failed = failed || failed_now;
-        const bool commit = judged && !failed_now && !converged_stagnant;
+        const bool commit = usable && !failed_now && !converged_stagnant;
 #pragma unroll
         for (int i = 0; i < W; ++i) P.u[i] = commit ? P.u[i] + delta[i] : P.u[i];
-        converged = converged || converged_stagnant || (commit && (eta_accept || small_first));
-        ndz_prev = commit ? ndz : real(0);
+        converged = converged || converged_stagnant || (commit && judged && (eta_accept || small_first));
+        ndz_prev = (commit && judged) ? ndz : real(0);
         prev_theta = (judged && history) ? theta : prev_theta;
     }
     theta_store = converged ? prev_theta : real(1);
@@ -391,6 +400,7 @@ struct DirkStageProblem {
 
     static constexpr bool cache = Cfg::cache_jac;
     Vec<L, (cache ? n * n : 1)> J;   // J(base + a u), refreshed per linear solve
+    Vec<L, n> kw;                    // 1 / Krylov tolerance weights (base is fixed)
 
     __device__ __forceinline__ void stage_state_l(L* Y) const {
 #pragma unroll
@@ -403,6 +413,11 @@ struct DirkStageProblem {
             L Y[n];
             stage_state_l(Y);
             Sys::jac(Y, params_l, drivers_l, J.data(), L(t));
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            L tol = L(Cfg::krylov_atol[i]) + L(Cfg::krylov_rtol[i]) * dabs(L(base[i]));
+            kw[i] = L(1) / pmax(tol, L(1e-16));
         }
     }
     __device__ __forceinline__ void jv(const L* v, L* out) const {
@@ -454,13 +469,13 @@ struct DirkStageProblem {
 #pragma unroll
         for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
     }
+    // Mean-square weighted norm; the weights' reciprocals are formed once per
+    // linear solve (a multiply per entry instead of a division).
     __device__ __forceinline__ L krylov_norm2(const L* vals) const {
         L nrm2 = L(0);
 #pragma unroll
         for (int i = 0; i < n; ++i) {
-            L tol = L(Cfg::krylov_atol[i]) + L(Cfg::krylov_rtol[i]) * dabs(L(base[i]));
-            tol = pmax(tol, L(1e-16));
-            const L r = dabs(vals[i]) / tol;
+            const L r = dabs(vals[i]) * kw[i];
             nrm2 += r * r;
         }
         return nrm2 * L(Cfg::inv_n);

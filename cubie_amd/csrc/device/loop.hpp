@@ -33,7 +33,63 @@ struct LoopCtx {
     Drivers<Cfg> drv;
 };
 
-template <typename Cfg, typename Sys, typename Stepper>
+// Recording buffers of the discrete adjoint's forward pass (the adjoint
+// kernels are in adjoint.hpp): per system, the size of every accepted step
+// and whether a save row / summary sample was taken at its end, and the
+// state and time every Cfg::adj_ckpt_every accepted steps (and at t0).
+template <typename Cfg>
+struct Tape {
+    using real = typename Cfg::real;
+    real* dt;                 // [max_steps][N]
+    unsigned char* flags;     // [max_steps][N]: bit 0 save row, bit 1 summary sample
+    real* ckpt;               // [max_ckpt][n][N]
+    double* ckpt_t;           // [max_ckpt][N]
+    long long max_steps, max_ckpt, N, run;
+    long long steps;
+    bool full;
+
+    template <typename Args>
+    __device__ __forceinline__ void bind(const Args& A, long long r) {
+        dt = static_cast<real*>(A.tape_dt);
+        flags = A.tape_flags;
+        ckpt = static_cast<real*>(A.checkpoints);
+        ckpt_t = A.checkpoint_t;
+        max_steps = A.max_steps;
+        max_ckpt = A.max_checkpoints;
+        N = A.fwd.n_runs;
+        run = r;
+        steps = 0;
+        full = false;
+    }
+
+    template <typename S>
+    __device__ __forceinline__ void checkpoint(long long c, const S& state, double t) {
+        if (c < max_ckpt) {
+#pragma unroll
+            for (int i = 0; i < Cfg::n; ++i) ckpt[(c * Cfg::n + i) * N + run] = state[i];
+            ckpt_t[c * N + run] = t;
+        } else {
+            full = true;
+        }
+    }
+
+    template <typename S>
+    __device__ __forceinline__ void step(real h, bool saved, bool sampled, const S& state_after,
+                                         double t_after) {
+        if (steps < max_steps) {
+            dt[steps * N + run] = h;
+            flags[steps * N + run] = (unsigned char)((saved ? 1 : 0) | (sampled ? 2 : 0));
+        } else {
+            full = true;
+        }
+        steps += 1;
+        if (steps % Cfg::adj_ckpt_every == 0)
+            checkpoint(steps / Cfg::adj_ckpt_every, state_after, t_after);
+    }
+};
+
+// Record = true: the adjoint's recording pass (fills a Tape as it goes).
+template <typename Cfg, typename Sys, typename Stepper, bool Record = false>
 struct Integrator {
     using real = typename Cfg::real;
     using nreal = typename Cfg::norm_real;
@@ -56,6 +112,7 @@ struct Integrator {
     int stagnant, update_idx, status, iteration_status;
     long long save_idx, summary_idx;
     bool first_step, prev_accepted, irrecoverable;
+    Tape<Cfg> tape;   // used only when Record
 
     __device__ __forceinline__ void begin(const cubie_solve_args& a, long long r) {
         run = r;
@@ -106,6 +163,7 @@ struct Integrator {
         if constexpr (Cfg::n_obs > 0)
             Sys::observables(state.data(), ctx.params.data(), ctx.drivers.data(),
                              ctx.observables.data(), t_prec);
+        if constexpr (Record) tape.checkpoint(0, state, t);
         if (settling == real(0)) {
             if constexpr (Cfg::save_regularly) next_save = real(next_save + Cfg::save_every);
             if constexpr (Cfg::summarise_regularly) next_update = real(Cfg::sample_summaries_every + next_update);
@@ -225,6 +283,9 @@ struct Integrator {
 
         do_save = do_save && accept;
         do_update = do_update && accept;
+        if constexpr (Record) {
+            if (accept) tape.step(dt_eff, do_save, do_update, state, t);
+        }
         if (do_save) {
             if constexpr (Cfg::save_regularly) next_save += Cfg::save_every;
             save_state<Cfg>(a, run, save_idx, state, ctx.observables, counters_since_save, t_prec);

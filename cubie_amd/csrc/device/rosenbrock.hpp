@@ -30,6 +30,7 @@ struct RosenbrockProblem {
     const L* drivers;
     const L* J;         // dense row-major Jacobian when cache_jac
     const L* Jd;        // Jacobian diagonal
+    const L* kw;        // 1 / Krylov tolerance weights (fixed per step)
     real t, h;
 
     __device__ __forceinline__ void prepare_linear() {}   // J is taken once per step
@@ -67,9 +68,7 @@ struct RosenbrockProblem {
         L nrm2 = L(0);
 #pragma unroll
         for (int i = 0; i < n; ++i) {
-            L tol = L(Cfg::krylov_atol[i]) + L(Cfg::krylov_rtol[i]) * dabs(state[i]);
-            tol = pmax(tol, L(1e-16));
-            const L r = dabs(vals[i]) / tol;
+            const L r = dabs(vals[i]) * kw[i];
             nrm2 += r * r;
         }
         return nrm2 * L(Cfg::inv_n);
@@ -150,7 +149,14 @@ struct RosenbrockStep {
             if constexpr (Cfg::has_error) error[i] = real(0);
         }
 
+        Vec<L, n> kw;
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            const L tol = L(Cfg::krylov_atol[i]) + L(Cfg::krylov_rtol[i]) * dabs(state_l[i]);
+            kw[i] = L(1) / pmax(tol, L(1e-16));
+        }
         Prob P;
+        P.kw = kw.data();
         P.state = state_l.data();
         P.params = li.params;
         P.drivers = li.drivers;

@@ -36,6 +36,11 @@ DEVICE_HEADERS = [
     "erk.hpp", "dirk.hpp", "firk.hpp", "rosenbrock.hpp", "euler_implicit.hpp", "loop.hpp",
 ]
 
+# Step adjoints of the discrete adjoint (adjoint.hpp), by family.
+FAMILY_ADJOINT = {"erk": "ErkAdjoint", "rosenbrock": "RosenbrockAdjoint"}
+# Largest state count of the Rosenbrock adjoint's dense stage solves.
+ROSENBROCK_ADJOINT_MAX_N = 8
+
 FAMILY_STEPPER = {
     "erk": "ERKStep", "dirk": "DIRKStep", "firk": "FIRKStep", "rosenbrock": "RosenbrockStep",
     "backwards_euler": "BackwardEulerStep", "backwards_euler_pc": "BackwardEulerStep",
@@ -116,6 +121,9 @@ class KernelSpec:
     fp_contract: bool = False
     deterministic_pow: bool = False
     block_size: int = 256
+    waves_per_eu: int = 0
+    adjoint: bool = False           # also emit the discrete-adjoint kernels
+    adj_ckpt_every: int = 32        # accepted steps per adjoint checkpoint
 
     # -- helpers ---------------------------------------------------------------
     @property
@@ -123,6 +131,16 @@ class KernelSpec:
         """Compensated mode of an implicit family: FP64 outer iteration,
         FP32 Krylov solves."""
         return self.compensated and self.algorithm.is_implicit
+
+    @property
+    def fp32_krylov(self) -> bool:
+        """Whether the Krylov solves run in FP32. In the mixed mode the
+        Newton-type families (DIRK, FIRK, backward Euler, Crank-Nicolson)
+        solve in FP32 and correct against an FP64 residual; a Rosenbrock
+        stage is a single linear solve with no outer correction, whose
+        accuracy FP32 cannot guarantee on stiff systems (the FP32 attainable
+        residual sits above the stopping test), so it stays in FP64."""
+        return self.mixed_implicit and self.algorithm.family != "rosenbrock"
 
     @property
     def work_precision(self):
@@ -153,7 +171,7 @@ class KernelSpec:
         a(f"  using real = {self._real_name()};")
         io_real = "double" if p is np.float64 else "float"
         a(f"  using io_real = {io_real};           // global arrays (system precision)")
-        lin_real = "float" if self.mixed_implicit else self._real_name()
+        lin_real = "float" if self.fp32_krylov else self._real_name()
         a(f"  using lin_real = {lin_real};          // Krylov solves")
         norm_real = "double" if (self.compensated or p is np.float64) else "float"
         a(f"  using norm_real = {norm_real};")
@@ -261,11 +279,13 @@ class KernelSpec:
             a(f"  static constexpr real gamma = {self._lit(tab.gamma)};")
             a(f"  static constexpr real gamma_stages[{st}] = {c_array(tab.gamma_stages, wp)};")
         a(f"  static constexpr bool be_predictor = {str(alg.family == 'backwards_euler_pc').lower()};")
+        a(f"  static constexpr int adj_ckpt_every = {int(self.adj_ckpt_every)};")
         # implicit solver settings
         im = self.implicit
         lin = {"minimal_residual": 0, "steepest_descent": 1, "bicgstab": 2}[im.linear_correction_type]
         a(f"  static constexpr int newton_max_iters = {int(im.newton_max_iters)};")
         a(f"  static constexpr int krylov_max_iters = {int(im.krylov_max_iters)};")
+        a(f"  static constexpr bool newton_partial = {'true' if self.fp32_krylov else 'false'};")
         a(f"  static constexpr int linear_solver = {lin};")
         a(f"  static constexpr int preconditioner_order = {int(im.preconditioner_order)};")
         # Small systems evaluate the dense Jacobian once per linear solve and
@@ -305,7 +325,7 @@ class KernelSpec:
         diag = [tab.a[i][i] for i in range(st)]
         later_explicit = not all(x != 0.0 for x in diag[1:])
         a(f"  static constexpr bool dirk_has_later_explicit = {str(later_explicit).lower()};")
-        red, flo = im.krylov_stopping(p, self.mixed_implicit)
+        red, flo = im.krylov_stopping(p, self.fp32_krylov)
         a(f"  static constexpr real krylov_reduction = {self._lit(red)};")
         a(f"  static constexpr real krylov_floor = {self._lit(flo)};")
         a("};")
@@ -338,7 +358,9 @@ class KernelSpec:
             parts.append(f.read())
         parts.append(self.config_struct())
         parts.append("using real = Cfg::real;")
-        gen = SystemCodegen(self.system, self.precision)
+        if self.adjoint:
+            self.check_adjoint()
+        gen = SystemCodegen(self.system, self.precision, adjoint=self.adjoint)
         parts.append(gen.emit())
         self.op_counts = gen.op_counts
         parts.append(self.plugin_hooks())
@@ -348,13 +370,49 @@ class KernelSpec:
                 parts.append(f.read())
         stepper = FAMILY_STEPPER[self.algorithm.family]
         bs = int(self.block_size)
+        attrs = f"__launch_bounds__({bs})"
+        if self.waves_per_eu > 0:
+            # occupancy floor: caps the per-lane VGPR budget at 512 / waves
+            attrs += f" __attribute__((amdgpu_waves_per_eu({int(self.waves_per_eu)})))"
         parts.append(
             f"using StepperT = cubie::{stepper}<Cfg, System>;\n"
-            f"extern \"C\" __global__ void __launch_bounds__({bs}) cubie_integrate(cubie_solve_args a) {{\n"
+            f"extern \"C\" __global__ void {attrs} cubie_integrate(cubie_solve_args a) {{\n"
             "  cubie::integrate_plain<Cfg, System, StepperT>(a);\n}\n"
-            f"extern \"C\" __global__ void __launch_bounds__({bs}) cubie_integrate_persistent(cubie_solve_args a) {{\n"
+            f"extern \"C\" __global__ void {attrs} cubie_integrate_persistent(cubie_solve_args a) {{\n"
             "  cubie::integrate_persistent<Cfg, System, StepperT>(a);\n}\n")
+        if self.adjoint:
+            with open(os.path.join(CSRC, "device", "adjoint.hpp")) as f:
+                parts.append("// ---- adjoint.hpp ----")
+                parts.append(f.read())
+            adj = FAMILY_ADJOINT[self.algorithm.family]
+            parts.append(
+                f"using AdjointT = cubie::{adj}<Cfg, System>;\n"
+                f"extern \"C\" __global__ void {attrs} cubie_integrate_record(cubie_adjoint_args a) {{\n"
+                "  cubie::integrate_record<Cfg, System, StepperT>(a);\n}\n"
+                f"extern \"C\" __global__ void {attrs} cubie_adjoint(cubie_adjoint_args a) {{\n"
+                "  cubie::adjoint_backward<Cfg, System, AdjointT>(a);\n}\n")
         return "\n".join(parts)
+
+    def check_adjoint(self) -> None:
+        """The discrete adjoint covers the explicit RK and Rosenbrock-W
+        families on FP32 / FP64 systems (not the compensated mode), with
+        state outputs and state summaries."""
+        fam = self.algorithm.family
+        if fam not in FAMILY_ADJOINT:
+            raise NotImplementedError(
+                f"the discrete adjoint supports explicit Runge-Kutta and Rosenbrock-W methods, "
+                f"not the {fam!r} family")
+        if fam == "rosenbrock" and self.system.num_states > ROSENBROCK_ADJOINT_MAX_N:
+            raise NotImplementedError(
+                f"the Rosenbrock adjoint solves its stages densely; at most "
+                f"{ROSENBROCK_ADJOINT_MAX_N} states")
+        if self.compensated:
+            raise NotImplementedError("the discrete adjoint does not support precision_mode='compensated'")
+
+    @property
+    def adjoint_stash(self) -> int:
+        """Stage values the backward pass keeps per replayed step."""
+        return self.algorithm.tableau.stage_count * self.system.num_states
 
     def compile_options(self) -> List[str]:
         opts = ["--offload-arch=gfx950", "-O3", "-std=c++17",

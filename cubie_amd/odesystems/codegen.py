@@ -15,6 +15,14 @@ emitted as HIP C++ for hiprtc:
                                    nonzero mask ``jac_nz``
 * ``jacp(y, p, d, Jp, t)``         dense df/dparams (row-major)
 
+and, for the discrete adjoint (``adjoint=True``):
+
+* ``vjp(y, p, d, w, gy, gp, t)``   gy = J^T w, gp = f_p^T w: the gradients of
+                                   w . f with respect to states and parameters
+* ``hvp(y, p, d, u, z, gy, gp, t)`` gradients of z . (J u) (second derivatives,
+                                   for the Rosenbrock stage matrix)
+* ``ftvjp(y, p, d, dd, z, gy, gp, t)`` gradients of z . df/dt
+
 Every function is a template on the scalar type ``real`` (the working
 precision of the caller: FP32 or FP64, and both in the mixed-precision mode,
 where residuals run in FP64 and Krylov products in FP32). Every array
@@ -207,9 +215,10 @@ def count_ops(exprs: Sequence[sp.Expr]) -> Dict[str, int]:
 class SystemCodegen:
     """Generate the ``System`` struct of device functions for one system."""
 
-    def __init__(self, system: SymbolicODE, precision=None):
+    def __init__(self, system: SymbolicODE, precision=None, adjoint: bool = False):
         self.system = system
         self.precision = precision or system.precision
+        self.adjoint = bool(adjoint)
         self.printer = HipPrinter(self.precision, system.user_functions)
         self.op_counts: Dict[str, Dict[str, int]] = {}
 
@@ -252,6 +261,57 @@ class SystemCodegen:
 
     def _unused(self, body: List[str]) -> List[str]:
         return body
+
+    def _grad_block(self, scalar: sp.Expr, key: str) -> List[str]:
+        s = self.system
+        targets = [(f"gy[{i}]", sp.diff(scalar, y)) for i, y in enumerate(s.state_symbols)]
+        targets += [(f"gp[{i}]", sp.diff(scalar, q)) for i, q in enumerate(s.parameter_symbols)]
+        return self._emit_block(targets, key)
+
+    def _emit_adjoint(self, dx, dfdt, dds) -> List[str]:
+        """vjp / hvp / ftvjp: gradients of weighted sums of f, J u and df/dt
+        with respect to the states and the parameters."""
+        s = self.system
+        n = s.num_states
+        sig = ("const real* __restrict__ y, const real* __restrict__ p, "
+               "const real* __restrict__ d, ")
+        out_sig = "real* __restrict__ gy, real* __restrict__ gp, real t"
+        parts = []
+        ws = [sp.Symbol(f"adj_w_{i}", real=True) for i in range(n)]
+        us = [sp.Symbol(f"adj_u_{i}", real=True) for i in range(n)]
+
+        def head(name, extra):
+            parts.append(f"  template <typename real> __device__ static __forceinline__ void {name}("
+                         f"{sig}{extra}{out_sig}) {{")
+            parts.extend(self._bindings())
+
+        def bind(syms, arr):
+            for i, sym in enumerate(syms):
+                parts.append(f"    [[maybe_unused]] const real {_cname(str(sym))} = {arr}[{i}];")
+
+        def tail():
+            parts.append("    (void)y; (void)p; (void)d; (void)gy; (void)gp; (void)t;")
+            parts.append("  }")
+
+        # vjp: grad of w . f
+        head("vjp", "const real* __restrict__ w, ")
+        bind(ws, "w")
+        parts.extend(self._grad_block(sp.Add(*[w * f for w, f in zip(ws, dx)]), "vjp"))
+        tail()
+        # hvp: grad of z . (J u)
+        head("hvp", "const real* __restrict__ u, const real* __restrict__ z, ")
+        bind(us, "u")
+        bind(ws, "z")
+        ju = [sp.Add(*[sp.diff(f, yj) * uj for yj, uj in zip(s.state_symbols, us)]) for f in dx]
+        parts.extend(self._grad_block(sp.Add(*[z * e for z, e in zip(ws, ju)]), "hvp"))
+        tail()
+        # ftvjp: grad of z . df/dt (driver derivatives dd fixed)
+        head("ftvjp", "const real* __restrict__ dd, const real* __restrict__ z, ")
+        bind(dds, "dd")
+        bind(ws, "z")
+        parts.extend(self._grad_block(sp.Add(*[z * e for z, e in zip(ws, dfdt)]), "ftvjp"))
+        tail()
+        return parts
 
     # -- functions -------------------------------------------------------------
     def emit(self) -> str:
@@ -336,5 +396,7 @@ class SystemCodegen:
         parts += self._emit_block(Jp, "jacp")
         parts.append("    (void)y; (void)p; (void)d; (void)Jp; (void)t;")
         parts.append("  }")
+        if self.adjoint:
+            parts += self._emit_adjoint(dx, dfdt, dds)
         parts.append("};")
         return "\n".join(parts) + "\n"

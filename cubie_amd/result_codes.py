@@ -3,7 +3,9 @@
 Same names and values as the reference's ``CUBIE_RESULT_CODES``; the device
 kernels set these bits in each run's int32 status word.
 ``MAX_LOOP_ITERS_EXCEEDED`` (reserved in the reference) is set by this
-engine's optional attempted-step cap.
+engine's optional attempted-step cap. ``ADJOINT_TAPE_FULL`` is this
+engine's own: the recording pass of the discrete adjoint ran out of tape
+or checkpoint space for that run.
 """
 
 from __future__ import annotations
@@ -22,6 +24,7 @@ class CUBIE_RESULT_CODES(IntFlag):
     STAGNATION = 64
     BICGSTAB_BREAKDOWN = 128
     NEWTON_DIVERGENCE = 256
+    ADJOINT_TAPE_FULL = 4096
 
 
 def decode_status_codes(status_codes: Optional[Iterable[int]]) -> Dict[int, List[str]]:

@@ -62,6 +62,30 @@ class SolveArgs(ctypes.Structure):
     ]
 
 
+class AdjointArgs(ctypes.Structure):
+    """Mirror of ``cubie_adjoint_args`` (include/cubie_hip.h)."""
+    _fields_ = [
+        ("fwd", SolveArgs),
+        ("checkpoints", ctypes.c_void_p),
+        ("checkpoint_t", ctypes.c_void_p),
+        ("tape_dt", ctypes.c_void_p),
+        ("tape_flags", ctypes.c_void_p),
+        ("tape_info", ctypes.c_void_p),
+        ("replay", ctypes.c_void_p),
+        ("replay_t", ctypes.c_void_p),
+        ("replay_stages", ctypes.c_void_p),
+        ("grad_state", ctypes.c_void_p),
+        ("grad_state_summ", ctypes.c_void_p),
+        ("grad_inits", ctypes.c_void_p),
+        ("grad_params", ctypes.c_void_p),
+        ("max_steps", ctypes.c_longlong),
+        ("max_checkpoints", ctypes.c_longlong),
+    ]
+
+
+ABI_VERSION = 2
+
+
 def build_runtime(force: bool = False) -> str:
     """Compile ``libcubie_hip.so`` (host code: hiprtc + module launch)."""
     if not force and os.path.exists(LIB_PATH) and os.path.getmtime(LIB_PATH) >= max(
@@ -116,7 +140,15 @@ def lib() -> ctypes.CDLL:
                                              ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                                              ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         L.cubie_hip_launch_timed.restype = ctypes.c_int
-        if L.cubie_hip_abi_version() != 1:
+        L.cubie_hip_launch_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(AdjointArgs),
+                                               ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                               ctypes.c_void_p]
+        L.cubie_hip_launch_adjoint.restype = ctypes.c_int
+        L.cubie_hip_launch_adjoint_timed.argtypes = [ctypes.c_void_p, ctypes.POINTER(AdjointArgs),
+                                                     ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.cubie_hip_launch_adjoint_timed.restype = ctypes.c_int
+        if L.cubie_hip_abi_version() != ABI_VERSION:
             raise CubieNativeError("libcubie_hip.so ABI version mismatch; rebuild it")
         _lib = L
         return L
@@ -177,6 +209,21 @@ class Module:
                                            ctypes.c_void_p(stream), ctypes.byref(ms)),
               "cubie_hip_launch_timed")
         return float(ms.value)
+
+    def launch_adjoint(self, args: AdjointArgs, grid: int, block: int, stream: int = 0,
+                       lds: int = 0, timed: bool = False):
+        """Launch cubie_integrate_record / cubie_adjoint; returns the kernel
+        time in ms when ``timed``."""
+        if timed:
+            ms = ctypes.c_float()
+            check(lib().cubie_hip_launch_adjoint_timed(self.handle, ctypes.byref(args), grid, block,
+                                                       lds, ctypes.c_void_p(stream),
+                                                       ctypes.byref(ms)),
+                  "cubie_hip_launch_adjoint_timed")
+            return float(ms.value)
+        check(lib().cubie_hip_launch_adjoint(self.handle, ctypes.byref(args), grid, block, lds,
+                                             ctypes.c_void_p(stream)), "cubie_hip_launch_adjoint")
+        return None
 
     def __del__(self):
         try:

@@ -25,7 +25,7 @@
 #ifndef CUBIE_HIP_H
 #define CUBIE_HIP_H
 
-#define CUBIE_ABI_VERSION 1
+#define CUBIE_ABI_VERSION 2
 
 /* Error codes */
 #define CUBIE_OK 0
@@ -54,7 +54,7 @@
  *   counters_out    [rows][4][n_runs] int32  (newton, krylov, steps, rejects)
  *   status_out      [n_runs] int32
  *   rtol_per_run    [n_runs] precision or null
- *   t_end_per_run   [n_runs] double (duration per run) or null
+ *   duration_per_run [n_runs] double (duration per run) or null
  *   save_times      [rows] double, shared save table, or null
  */
 typedef struct cubie_solve_args {
@@ -86,6 +86,50 @@ typedef struct cubie_solve_args {
     int chunk_runs;            /* runs taken per queue grab (persistent kernel) */
     int reserved;
 } cubie_solve_args;
+
+/*
+ * Arguments of the discrete-adjoint kernels (no counterpart in the
+ * reference, which has no adjoint). `fwd` is the forward solve's argument
+ * block: the recording pass cubie_integrate_record fills its outputs like
+ * cubie_integrate and also writes the tape; the backward pass cubie_adjoint
+ * reads the same block (parameters, drivers, the forward outputs that the
+ * max / min / rms summaries are differentiated through) plus the tape and
+ * the output gradients, and writes grad_inits / grad_params.
+ *
+ *   checkpoints     [max_checkpoints][n_states][n_runs]  precision
+ *   checkpoint_t    [max_checkpoints][n_runs]            double
+ *   tape_dt         [max_steps][n_runs]                   precision (accepted h)
+ *   tape_flags      [max_steps][n_runs] uint8: bit 0 a save row, bit 1 a
+ *                   summary sample, taken at the end of that step
+ *   tape_info       [3][n_runs] int32: accepted steps, save rows, samples
+ *   replay          [K + 1][n_states][n_runs]             precision
+ *   replay_t        [K + 1][n_runs]                       double
+ *   replay_stages   [K][stash][n_runs]                    precision
+ *   grad_state      dL/d state_out, laid out like state_out, or null
+ *   grad_state_summ dL/d state_summ_out, laid out like it, or null
+ *   grad_inits      [n_states][n_runs]                    precision (output)
+ *   grad_params     [n_params][n_runs]                    precision (output)
+ * K (accepted steps per checkpoint) and stash (stage values kept per step)
+ * are compile-time constants of the module. A run whose tape or checkpoint
+ * buffer overflows gets status bit 4096 and no gradient.
+ */
+typedef struct cubie_adjoint_args {
+    cubie_solve_args fwd;
+    void* checkpoints;
+    double* checkpoint_t;
+    void* tape_dt;
+    unsigned char* tape_flags;
+    int* tape_info;
+    void* replay;
+    double* replay_t;
+    void* replay_stages;
+    const void* grad_state;
+    const void* grad_state_summ;
+    void* grad_inits;
+    void* grad_params;
+    long long max_steps;
+    long long max_checkpoints;
+} cubie_adjoint_args;
 
 #ifndef CUBIE_DEVICE_COMPILE
 #include <stddef.h>
@@ -146,6 +190,19 @@ int cubie_hip_launch_timed(cubie_module_t module, const cubie_solve_args* args,
                            unsigned grid, unsigned block, unsigned dynamic_lds,
                            void* stream, float* elapsed_ms);
 
+/*
+ * Launch a discrete-adjoint kernel of `module` (cubie_integrate_record or
+ * cubie_adjoint, resolved by cubie_hip_load) with the argument block above.
+ * No counterpart in the reference (it has no adjoint); the timed form
+ * brackets the launch with hipEvents on `stream` like
+ * cubie_hip_launch_timed.
+ */
+int cubie_hip_launch_adjoint(cubie_module_t module, const cubie_adjoint_args* args,
+                             unsigned grid, unsigned block, unsigned dynamic_lds,
+                             void* stream);
+int cubie_hip_launch_adjoint_timed(cubie_module_t module, const cubie_adjoint_args* args,
+                                   unsigned grid, unsigned block, unsigned dynamic_lds,
+                                   void* stream, float* elapsed_ms);
 #ifdef __cplusplus
 }
 #endif

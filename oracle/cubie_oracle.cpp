@@ -160,6 +160,10 @@ struct OracleConfig {
     int64_t* work_out;
     // mixed precision (implicit families): FP64 outer iteration, FP32 Krylov
     int32_t mixed;
+    // per-run relative tolerance (replaces every rtol[i]) and per-run
+    // duration, n_runs entries each; null when not used
+    const double* rtol_per_run;
+    const double* duration_per_run;
 };
 
 }  // extern "C"
@@ -387,6 +391,8 @@ struct ControllerState {
     const OracleConfig* cfg;
     int n;
     T atol[MAXN], rtol[MAXN];
+    T rtol_run;
+    bool per_run_rtol;
     T history[2];
     T kp, ki, kd_e, expo, safety, min_gain, max_gain, dt_min, dt_max;
     T deadband_min, deadband_max, inv_n, gain_numerator;
@@ -397,6 +403,8 @@ struct ControllerState {
         cfg = c;
         n = n_;
         for (int i = 0; i < n; ++i) { atol[i] = T(c->atol[i]); rtol[i] = T(c->rtol[i]); }
+        per_run_rtol = false;
+        rtol_run = T(0);
         history[0] = history[1] = T(0);
         const int k = c->order;
         safety = T(c->safety);
@@ -425,7 +433,8 @@ struct ControllerState {
         T nrm2 = T(0);
         for (int i = 0; i < n; ++i) {
             T ei = std::max(std::fabs(error[i]), T(1e-16));
-            T tol = atol[i] + rtol[i] * std::max(std::fabs(state[i]), std::fabs(state_prev[i]));
+            T rt = per_run_rtol ? rtol_run : rtol[i];
+            T tol = atol[i] + rt * std::max(std::fabs(state[i]), std::fabs(state_prev[i]));
             T ratio = ei / tol;
             nrm2 += ratio * ratio;
         }
@@ -738,8 +747,13 @@ static int32_t run_loop(const OracleConfig* cfg, Problem<T>& P, Stepper& stepper
 
     ControllerState<T> ctrl;
     ctrl.init(cfg, n);
+    if (cfg->rtol_per_run) {
+        ctrl.per_run_rtol = true;
+        ctrl.rtol_run = T(cfg->rtol_per_run[run]);
+    }
 
-    T duration = T(cfg->duration), settling = T(cfg->warmup), t0 = T(cfg->t0);
+    T duration = T(cfg->duration_per_run ? cfg->duration_per_run[run] : cfg->duration);
+    T settling = T(cfg->warmup), t0 = T(cfg->t0);
     T save_stop = T(cfg->save_stop), summary_stop = T(cfg->summary_stop);
     T save_every = T(cfg->save_every);
     T sample_every = T(cfg->sample_summaries_every);

@@ -97,6 +97,8 @@ class OracleConfig(ctypes.Structure):
         ("be_predictor", ctypes.c_int32),
         ("work_out", ctypes.POINTER(ctypes.c_int64)),
         ("mixed", ctypes.c_int32),
+        ("rtol_per_run", _D),
+        ("duration_per_run", _D),
     ]
 
 
@@ -211,13 +213,15 @@ class OracleRun:
             return a
 
         # The mixed-precision (compensated) mode of an implicit family keeps the
-        # loop in FP64 and solves the Krylov systems in FP32; its inputs and
-        # outputs stay in the system precision.
+        # loop in FP64 and, for the Newton-type families, solves the Krylov
+        # systems in FP32 (Rosenbrock stages, which have no outer correction,
+        # stay in FP64); inputs and outputs stay in the system precision.
         self.mixed = (getattr(k, "precision_mode", "plain") == "compensated"
                       and alg.is_implicit)
+        fp32_krylov = self.mixed and alg.family != "rosenbrock"
         self.io_dtype = prec
         cfg.precision_bits = 64 if (self.mixed or prec is np.float64) else 32
-        cfg.mixed = int(self.mixed)
+        cfg.mixed = int(fp32_krylov)
         cfg.system_id = SYSTEM_IDS[system_id]
         cfg.n_states = sysm.num_states
         cfg.n_params = sysm.num_parameters
@@ -315,7 +319,7 @@ class OracleRun:
         cfg.newton_rtol_s = float(np.min(im.newton_rtol))
         cfg.krylov_atol_s = float(np.min(im.krylov_atol))
         cfg.krylov_rtol_s = float(np.min(im.krylov_rtol))
-        cfg.krylov_residual_reduction, cfg.krylov_floor = im.krylov_stopping(prec, self.mixed)
+        cfg.krylov_residual_reduction, cfg.krylov_floor = im.krylov_stopping(prec, fp32_krylov)
         if callbacks is not None:
             cfg.rhs_cb, cfg.obs_cb, cfg.jvp_cb = callbacks[:3]
             cfg.dfdt_cb = callbacks[3] if len(callbacks) > 3 else DFDT_CB()
@@ -354,10 +358,24 @@ class OracleRun:
         self.dur_p = dur_p
 
     def solve(self, inits: np.ndarray, params: np.ndarray, threads: int = 1,
-              runs: Optional[range] = None) -> Dict[str, np.ndarray]:
+              runs: Optional[range] = None, rtol_per_run=None,
+              duration_per_run=None) -> Dict[str, np.ndarray]:
+        """Integrate every run (or ``runs``). ``rtol_per_run`` replaces the
+        relative tolerance of each run (rounded to the system precision, as
+        the device reads it); ``duration_per_run`` gives each run its own
+        duration (the outputs are sized by the OracleRun's duration)."""
         inits = np.ascontiguousarray(np.asarray(inits, dtype=self.io_dtype), dtype=np.float64)
         params = np.ascontiguousarray(np.asarray(params, dtype=self.io_dtype), dtype=np.float64)
         N = inits.shape[1]
+        keep = []
+        if rtol_per_run is not None:
+            rt = np.ascontiguousarray(np.asarray(rtol_per_run, dtype=self.io_dtype), dtype=np.float64)
+            keep.append(rt)
+            self.cfg.rtol_per_run = _dp(rt)
+        if duration_per_run is not None:
+            dpr = np.ascontiguousarray(duration_per_run, dtype=np.float64)
+            keep.append(dpr)
+            self.cfg.duration_per_run = _dp(dpr)
         shapes = self.kernel.output_shapes(self.timing, self.dur_p, N)
         out = {}
         for key, shp in shapes.items():
@@ -395,6 +413,8 @@ class OracleRun:
         # per-run tallies: rhs, jvp, -, dfdt, krylov iterations, newton iterations
         res["work"] = work_tally
         self.cfg.work_out = ctypes.POINTER(ctypes.c_int64)()
+        self.cfg.rtol_per_run = _D()
+        self.cfg.duration_per_run = _D()
         return res
 
 

@@ -343,6 +343,13 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         T n2 = Pb.correction_norm2(delta);
         T ndz = T(std::sqrt(n2));
         bool judged = lin == ST_SUCCESS;
+        // Mixed mode only (not in the reference): an FP32 solve that stops
+        // short at the iteration cap or a BiCGSTAB breakdown still yields a
+        // correction; it is applied, but neither ends the iteration nor
+        // enters the contraction estimate.
+        bool partial = cfg->mixed && !judged &&
+                       (lin == ST_LINEAR_MAX_ITERS || lin == ST_BICGSTAB_BREAKDOWN);
+        bool usable = judged || partial;
         bool hist = ndz_prev > T(0);
         T nps = hist ? ndz_prev : T(1);
         T theta = hist ? std::max(decay * prev_theta, ndz / nps) : prev_theta;
@@ -350,14 +357,14 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         bool eta_ok = theta < T(1) && theta * ndz < kappa * (T(1) - theta);
         bool nonfinite = !(n2 <= huge);
         bool stagnant = judged && hist && std::fabs(theta - T(1)) <= stag;
-        bool diverging = judged && ((hist && theta > div) || nonfinite);
+        bool diverging = (judged && hist && theta > div) || (usable && nonfinite);
         bool conv_stag = stagnant && ndz <= T(1) && !diverging;
         bool failed_now = diverging || (stagnant && ndz > T(1));
         failed = failed || failed_now;
-        bool commit = judged && !failed_now && !conv_stag;
+        bool commit = usable && !failed_now && !conv_stag;
         for (int i = 0; i < W; ++i) Pb.u[i] = commit ? Pb.u[i] + delta[i] : Pb.u[i];
-        conv = conv || conv_stag || (commit && (eta_ok || small_first));
-        ndz_prev = commit ? ndz : T(0);
+        conv = conv || conv_stag || (commit && judged && (eta_ok || small_first));
+        ndz_prev = (commit && judged) ? ndz : T(0);
         prev_theta = (judged && hist) ? theta : prev_theta;
     }
     theta_store = conv ? prev_theta : T(1);

@@ -24,7 +24,8 @@ def test_header_declares_the_boundary():
     names = declared_functions()
     for required in ("cubie_hip_compile", "cubie_hip_load", "cubie_hip_launch",
                      "cubie_hip_launch_timed", "cubie_hip_kernel_info", "cubie_hip_last_error",
-                     "cubie_hip_abi_version", "cubie_hip_unload", "cubie_hip_free_code"):
+                     "cubie_hip_abi_version", "cubie_hip_unload", "cubie_hip_free_code",
+                     "cubie_hip_launch_adjoint", "cubie_hip_launch_adjoint_timed"):
         assert required in names
 
 
@@ -38,7 +39,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_buffer():
     from cubie_amd.runtime import native
     lib = native.lib()
-    assert lib.cubie_hip_abi_version() == 1
+    assert lib.cubie_hip_abi_version() == native.ABI_VERSION == 2
     buf = ctypes.create_string_buffer(256)
     assert lib.cubie_hip_last_error(buf, 256) >= 0
 
@@ -53,6 +54,18 @@ def test_solve_args_layout_matches_header():
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     fields = re.findall(r"\b([a-z_0-9]+)\s*;", body)
     assert [f[0] for f in SolveArgs._fields_] == fields
+
+
+def test_adjoint_args_layout_matches_header():
+    from cubie_amd.runtime.native import AdjointArgs, SolveArgs
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct cubie_adjoint_args"):]
+    body = body[: body.index("} cubie_adjoint_args;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"\b([a-z_0-9]+)\s*;", body)
+    assert [f[0] for f in AdjointArgs._fields_] == fields
+    # the forward block is embedded first, by value
+    assert AdjointArgs.fwd.offset == 0 and AdjointArgs.fwd.size == ctypes.sizeof(SolveArgs)
 
 
 def test_hiprtc_compiles_generated_kernel(tmp_path, monkeypatch):

@@ -57,7 +57,7 @@ def test_lorenz_dopri5_fp64_bitwise_parity(oracle_module):
     """
     n = 1024
     inits, params = lorenz_batch(n)
-    solver = ca.Solver(lorenz_system(np.float64), algorithm="dormand-prince-54",
+    solver = ca.Solver(fp_contract=False, system=lorenz_system(np.float64), algorithm="dormand-prince-54",
                        atol=1e-8, rtol=1e-6, save_every=0.01, deterministic_pow=True,
                        output_types=["state", "time", "iteration_counters"])
     res, ref = run_both(solver, oracle_module, "lorenz", inits, params, 10.0)
@@ -79,7 +79,7 @@ def test_lorenz_dopri5_fp64_native_pow(oracle_module):
     """
     n = 1024
     inits, params = lorenz_batch(n)
-    solver = ca.Solver(lorenz_system(np.float64), algorithm="dormand-prince-54",
+    solver = ca.Solver(fp_contract=False, system=lorenz_system(np.float64), algorithm="dormand-prince-54",
                        atol=1e-8, rtol=1e-6, save_every=0.01,
                        output_types=["state", "time", "iteration_counters"])
     res, ref = run_both(solver, oracle_module, "lorenz", inits, params, 10.0)
@@ -96,7 +96,7 @@ def test_fixed_step_erk_fp64_bitwise(oracle_module, alg):
     """Errorless tableaus run fixed-step; FP64 results equal the oracle to 1e-12."""
     n = 256
     inits, params = lorenz_batch(n, seed=3)
-    solver = ca.Solver(lorenz_system(np.float64), algorithm=alg, dt=0.001,
+    solver = ca.Solver(fp_contract=False, system=lorenz_system(np.float64), algorithm=alg, dt=0.001,
                        save_every=0.05, output_types=["state", "iteration_counters"])
     res, ref = run_both(solver, oracle_module, "lorenz", inits, params, 2.0)
     err = normwise_rel(res.state, ref["state"])
@@ -109,7 +109,7 @@ def test_fixed_step_erk_fp64_bitwise(oracle_module, alg):
 def test_adaptive_erk_fp64(oracle_module, alg):
     n = 256
     inits, params = lorenz_batch(n, seed=4)
-    solver = ca.Solver(lorenz_system(np.float64), algorithm=alg, atol=1e-8, rtol=1e-6,
+    solver = ca.Solver(fp_contract=False, system=lorenz_system(np.float64), algorithm=alg, atol=1e-8, rtol=1e-6,
                        save_every=0.1, deterministic_pow=True,
                        output_types=["state", "iteration_counters"])
     res, ref = run_both(solver, oracle_module, "lorenz", inits, params, 2.0)
@@ -122,7 +122,7 @@ def test_controllers_fp64(oracle_module, controller):
     n = 128
     inits, params = lorenz_batch(n, seed=5)
     extra = {"kd": 0.1} if controller == "pid" else {}
-    solver = ca.Solver(lorenz_system(np.float64), algorithm="dormand-prince-54",
+    solver = ca.Solver(fp_contract=False, system=lorenz_system(np.float64), algorithm="dormand-prince-54",
                        step_controller=controller, atol=1e-7, rtol=1e-5, dt=0.01,
                        save_every=0.1, deterministic_pow=True,
                        output_types=["state", "iteration_counters"], **extra)
@@ -136,7 +136,7 @@ def test_lorenz_fp32_matches_oracle(oracle_module):
     """Float32 stage arithmetic: oracle and kernel both run in float32."""
     n = 512
     inits, params = lorenz_batch(n, seed=6)
-    solver = ca.Solver(lorenz_system(np.float32), algorithm="dormand-prince-54",
+    solver = ca.Solver(fp_contract=False, system=lorenz_system(np.float32), algorithm="dormand-prince-54",
                        atol=1e-6, rtol=1e-4, save_every=0.01, deterministic_pow=True,
                        output_types=["state", "time", "iteration_counters"])
     res, ref = run_both(solver, oracle_module, "lorenz", inits, params, 1.0)

@@ -1,0 +1,103 @@
+"""Uneven batches on the GPU: per-system relative tolerance and end time,
+the persistent work-queue kernel, and the launch-shape autotuner.
+
+The batch is the Van der Pol oscillator with mu spread over four decades, a
+per-system end time and a per-system rtol, integrated by DOPRI5; only the
+final state is kept. The oracle takes the same per-run arrays."""
+
+import numpy as np
+import pytest
+
+import cubie_amd as ca
+from systems import van_der_pol, normwise_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def uneven_batch(n, seed, mu_max=100.0, t_max=20.0):
+    rng = np.random.default_rng(seed)
+    mu = np.exp(rng.uniform(np.log(0.1), np.log(mu_max), n))[None, :]
+    t_end = rng.uniform(1.0, t_max, n)
+    inits = rng.uniform(-2.0, 2.0, (2, n))
+    rtol = np.exp(rng.uniform(np.log(1e-8), np.log(1e-4), n))
+    return inits, mu, t_end, rtol
+
+
+KW = dict(algorithm="dormand-prince-54", atol=1e-10, rtol=1e-6, dt_min=1e-10, dt_max=1.0,
+          output_types=["state", "iteration_counters"])
+
+
+def test_uneven_fp64_matches_oracle(oracle_module):
+    """Per-run rtol and end time: every run's status, step count and final
+    state against the oracle given the same per-run arrays."""
+    n = 300
+    inits, mu, t_end, rtol = uneven_batch(n, seed=51)
+    solver = ca.Solver(van_der_pol(np.float64), deterministic_pow=True, **KW)
+    res = solver.solve(inits, mu, duration=float(t_end.max()), rtol_per_run=rtol,
+                       duration_per_run=t_end, nan_error_trajectories=False)
+    orc = oracle_module.OracleRun(solver.kernel, "van_der_pol", float(t_end.max())).solve(
+        inits, mu, threads=8, rtol_per_run=rtol, duration_per_run=t_end)
+    assert np.array_equal(res.status_codes, orc["status_codes"])
+    assert np.all(res.status_codes == 0)
+    steps_gpu = res.iteration_counters[:, 2, :].sum(axis=0)
+    steps_orc = orc["iteration_counters"][:, 2, :].sum(axis=0)
+    assert np.array_equal(steps_gpu, steps_orc)
+    err = normwise_rel(res.state[-1], orc["state"][-1])
+    print("uneven vdp fp64 final-state normwise rel err", err)
+    assert err <= 1e-12
+
+
+def test_uneven_persistent_matches_plain():
+    """The work-queue kernel hands systems to lanes in a different order;
+    the results are identical to one-lane-per-system."""
+    n = 1000       # several refills per lane at the smallest grid
+    inits, mu, t_end, rtol = uneven_batch(n, seed=52)
+    out = {}
+    for mode in ("plain", "persistent"):
+        s = ca.Solver(van_der_pol(np.float64), kernel_mode=mode, blocks_per_cu=1, **KW)
+        out[mode] = s.solve(inits, mu, duration=float(t_end.max()), rtol_per_run=rtol,
+                            duration_per_run=t_end, nan_error_trajectories=False)
+    assert np.array_equal(out["plain"].state, out["persistent"].state)
+    assert np.array_equal(out["plain"].iteration_counters, out["persistent"].iteration_counters)
+    assert np.array_equal(out["plain"].status_codes, out["persistent"].status_codes)
+
+
+def test_uneven_compensated_close_to_fp64(oracle_module):
+    """FP32 inputs, FP32 stages with a compensated state and an FP64 error
+    norm: the final states stay within the tolerance-level distance of the
+    FP64 oracle on the same (FP32-rounded) inputs."""
+    n = 300
+    inits, mu, t_end, rtol = uneven_batch(n, seed=53)
+    inits32, mu32, rtol32 = (x.astype(np.float32) for x in (inits, mu, rtol))
+    s32 = ca.Solver(van_der_pol(np.float32), precision_mode="compensated",
+                    kernel_mode="persistent", **KW)
+    res = s32.solve(inits32, mu32, duration=float(t_end.max()), rtol_per_run=rtol32,
+                    duration_per_run=t_end, nan_error_trajectories=False)
+    assert np.all(res.status_codes == 0), res.status_messages
+    s64 = ca.Solver(van_der_pol(np.float64), **KW)
+    orc = oracle_module.OracleRun(s64.kernel, "van_der_pol", float(t_end.max())).solve(
+        inits32.astype(np.float64), mu32.astype(np.float64), threads=8,
+        rtol_per_run=rtol32.astype(np.float64), duration_per_run=t_end)
+    a, b = res.state[-1].astype(np.float64), orc["state"][-1]
+    e = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+    print("uneven vdp compensated: median", np.median(e), "max", e.max())
+    # each run against its own rtol: global error of an rtol-controlled run
+    # is a small multiple of rtol over these horizons
+    assert np.all(e <= 1e3 * np.maximum(rtol32, 1e-6))
+    assert np.median(e) <= 1e-4
+
+
+def test_autotune_picks_a_timed_candidate():
+    from cubie_amd.batchsolving.autotune import autotune
+    n = 4096
+    inits, mu, t_end, rtol = uneven_batch(n, seed=54, mu_max=10.0, t_max=5.0)
+    s = ca.Solver(van_der_pol(np.float64), kernel_mode="persistent", **KW)
+    best = autotune(s, inits, mu, float(t_end.max()), sample_runs=2048, block_sizes=(64, 256),
+                    waves_per_eu=(0,), blocks_per_cu=(1, 2), rtol_per_run=rtol,
+                    duration_per_run=t_end)
+    assert len(best["table"]) == 4
+    assert best["ms"] == min(r["ms"] for r in best["table"])
+    assert (s.kernel.block_size, s.kernel.blocks_per_cu) == (best["block_size"], best["blocks_per_cu"])
+    res = s.solve(inits, mu, duration=float(t_end.max()), rtol_per_run=rtol,
+                  duration_per_run=t_end, nan_error_trajectories=False)
+    assert np.all(res.status_codes == 0)

@@ -1,0 +1,87 @@
+"""Host-side checks of the uneven-batch machinery (no GPU): the oracle's
+per-run tolerance and end time, the autotuner's candidate space and memo
+key, and the occupancy attribute in the generated source."""
+
+import numpy as np
+import pytest
+
+import cubie_amd as ca
+from cubie_amd.batchsolving.autotune import candidates
+from systems import van_der_pol
+
+KW = dict(algorithm="dormand-prince-54", atol=1e-10, rtol=1e-6, dt_min=1e-10, dt_max=1.0,
+          output_types=["state", "iteration_counters"])
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    mu = np.exp(rng.uniform(np.log(0.1), np.log(20.0), n))[None, :]
+    inits = rng.uniform(-2.0, 2.0, (2, n))
+    return inits, mu
+
+
+def test_oracle_per_run_rtol_equal_to_scalar_is_identity(oracle_module):
+    inits, mu = _batch(16, 1)
+    s = ca.Solver(van_der_pol(np.float64), **KW)
+    run = oracle_module.OracleRun(s.kernel, "van_der_pol", 5.0)
+    a = run.solve(inits, mu)
+    b = run.solve(inits, mu, rtol_per_run=np.full(16, 1e-6))
+    assert np.array_equal(a["state"], b["state"])
+    assert np.array_equal(a["iteration_counters"], b["iteration_counters"])
+
+
+def test_oracle_per_run_rtol_matches_separate_solves(oracle_module):
+    """Each run with its own rtol equals a solve of that run alone with the
+    scalar rtol set to the same value."""
+    inits, mu = _batch(6, 2)
+    rt = np.array([1e-8, 1e-7, 1e-6, 1e-5, 1e-4, 3e-6])
+    s = ca.Solver(van_der_pol(np.float64), **KW)
+    both = oracle_module.OracleRun(s.kernel, "van_der_pol", 5.0).solve(inits, mu, rtol_per_run=rt)
+    for i, r in enumerate(rt):
+        si = ca.Solver(van_der_pol(np.float64), **dict(KW, rtol=float(r)))
+        one = oracle_module.OracleRun(si.kernel, "van_der_pol", 5.0).solve(
+            inits[:, i:i + 1], mu[:, i:i + 1])
+        assert np.array_equal(both["state"][..., i], one["state"][..., 0])
+    # tighter tolerance, more steps
+    steps = both["iteration_counters"][:, 2, :].sum(axis=0)
+    assert steps[0] > steps[4]
+
+
+def test_oracle_per_run_duration_matches_separate_solves(oracle_module):
+    inits, mu = _batch(4, 3)
+    dur = np.array([1.0, 2.5, 4.0, 0.75])
+    s = ca.Solver(van_der_pol(np.float64), **KW)
+    both = oracle_module.OracleRun(s.kernel, "van_der_pol", 4.0).solve(
+        inits, mu, duration_per_run=dur)
+    for i, d in enumerate(dur):
+        one = oracle_module.OracleRun(s.kernel, "van_der_pol", float(d)).solve(
+            inits[:, i:i + 1], mu[:, i:i + 1])
+        assert np.array_equal(both["state"][-1][..., i], one["state"][-1][..., 0])
+
+
+def test_autotune_candidates():
+    c = candidates((64, 256), (0, 2), (1, 2, 4), "persistent", 4)
+    assert len(c) == 12 and (64, 0, 1) in c and (256, 2, 4) in c
+    # the plain kernel's grid covers the batch: no blocks-per-CU choice
+    c = candidates((64, 256), (0, 2), (1, 2, 4), "plain", 4)
+    assert c == [(64, 0, 4), (64, 2, 4), (256, 0, 4), (256, 2, 4)]
+
+
+def test_config_key_ignores_launch_shape():
+    s = ca.Solver(van_der_pol(np.float64), **KW)
+    k = s.kernel
+    key = k.config_key(5.0)
+    k.block_size, k.waves_per_eu = 64, 2
+    assert k.config_key(5.0) == key
+    assert (k.block_size, k.waves_per_eu) == (64, 2)
+    s2 = ca.Solver(van_der_pol(np.float64), **dict(KW, rtol=1e-7))
+    assert s2.kernel.config_key(5.0) != key
+    assert k.config_key(5.0, per_run_rtol=True) != key
+
+
+def test_waves_per_eu_in_source():
+    s = ca.Solver(van_der_pol(np.float64), waves_per_eu=2, **KW)
+    src = s.kernel.spec_for(5.0).source()
+    assert src.count("amdgpu_waves_per_eu(2)") == 2      # both kernels
+    s0 = ca.Solver(van_der_pol(np.float64), **KW)
+    assert "amdgpu_waves_per_eu" not in s0.kernel.spec_for(5.0).source()
