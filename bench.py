@@ -129,6 +129,46 @@ def hh_workload(n, precision=np.float64, duration=500.0, dt=2.0 ** -6):
                        "n_systems": n}}
 
 
+def hh6_system(precision):
+    """Hodgkin-Huxley with the reversal potentials as parameters as well
+    (six parameters per system: gNa, gK, gL, ENa, EK, EL)."""
+    import cubie_amd as ca
+    return ca.create_ODE_system(
+        HH_EQUATIONS, states={"V": -65.0, "m": 0.0529, "h": 0.5961, "n": 0.3177},
+        parameters={"gNa": 120.0, "gK": 36.0, "gL": 0.3, "ENa": 50.0, "EK": -77.0,
+                    "EL": -54.387},
+        constants={"Cm": 1.0}, drivers=["I_ext"], precision=precision, name="hodgkin_huxley6")
+
+
+def hh_adjoint_workload(n, precision=np.float64, duration=500.0, algorithm="dormand-prince-54"):
+    """Gradient workload: the HH batch with six parameters and four initial
+    values per system, loss = sum over systems of (mean V over the run -
+    target)^2 with seeded targets, adaptive (rtol = atol = 1e-6) DOPRI5 or
+    ROS3P, checkpoints every 32 accepted steps, mean V sampled every 0.5 ms."""
+    rng = np.random.default_rng(2026)
+    base = np.array([120.0, 36.0, 0.3, 50.0, -77.0, -54.387])[:, None]
+    params = base * rng.uniform(0.9, 1.1, (6, n))
+    inits = np.tile(np.array([[-65.0], [0.0529], [0.5961], [0.3177]]), (1, n))
+    inits[0] = rng.uniform(-70.0, -60.0, n)
+    target = rng.normal(-60.0, 2.0, n)
+    drng = np.random.default_rng(7)
+    t = np.arange(1000) * 0.5
+    current = 10.0 * np.sin(2.0 * np.pi * t / 50.0) + drng.normal(0.0, 2.0, 1000)
+    solver_kw = dict(algorithm=algorithm, atol=1e-6, rtol=1e-6, summarise_every=duration,
+                     sample_summaries_every=0.5, summarise_variables=["V"],
+                     output_types=["mean"])
+    if algorithm == "ros3p":
+        solver_kw["linear_correction_type"] = "bicgstab"
+    return {"name": "hh_adjoint", "system_factory": hh6_system, "inits": inits,
+            "params": params, "target": target, "duration": duration, "solver_kw": solver_kw,
+            "oracle_system": "hodgkin_huxley6",
+            "drivers": {"I_ext": current, "dt": 0.5, "order": 1, "wrap": False},
+            "config": {"workload": f"Hodgkin-Huxley discrete adjoint, {algorithm}, 6 params + 4 "
+                                   f"initial values per system, loss = sum (mean V - target)^2, "
+                                   f"t in [0,{duration:g}] ms, rtol 1e-6, checkpoint every 32 "
+                                   "accepted steps", "n_systems": n}}
+
+
 def vdp_system(precision):
     import cubie_amd as ca
     return ca.create_ODE_system(
@@ -345,12 +385,151 @@ def measure(work, solver, steps, warmup):
             "spec": info["spec"], "kernel_info": info["module"].info()}
 
 
+def adjoint_flops_per_step(spec):
+    """Algorithmic flops of one accepted step of the backward kernel (the
+    replay of the step plus its adjoint), from the generated functions'
+    IR counts: explicit RK s (f + vjp) plus the stage combinations;
+    Rosenbrock two Jacobians and LU factorisations, s (f + vjp + hvp + ftvjp)
+    and the triangular solves."""
+    ops = spec.op_counts
+    n = spec.system.num_states
+    tab = spec.algorithm.tableau
+    s_ = tab.stage_count
+    nnz = int(np.count_nonzero(np.array(tab.a)))
+    if spec.algorithm.family == "erk":
+        return (s_ * (_f(ops, "dxdt") + _f(ops, "vjp")) + 4 * n * nnz + 6 * n * s_)
+    lu = 2 * (2 * n ** 3 // 3 + 2 * n * n)
+    return (2 * (_f(ops, "jac") + _f(ops, "dfdt")) + lu
+            + s_ * (_f(ops, "dxdt") + _f(ops, "vjp") + _f(ops, "hvp") + _f(ops, "ftvjp")
+                    + 6 * n * n) + 4 * n * nnz)
+
+
+def adjoint_main(args):
+    """Gradient throughput of the discrete adjoint (record + backward) next
+    to the forward-only FP64 kernel on the same batch."""
+    import torch
+    from cubie_amd.batchsolving import adjoint as adj
+    n_sys = args.n if args.n_given else 262144
+    names = args.variants.split(",") if args.variants else ["dormand-prince-54", "ros3p"]
+    kw = {} if args.duration is None else {"duration": args.duration}
+    out = {}
+    for alg in names:
+        work = hh_adjoint_workload(n_sys, algorithm=alg, **kw)
+        solver = make_solver(work, np.float64, "plain", "plain")
+        k = solver.kernel
+        if args.compile_only:
+            k.spec_for(work["duration"])
+            k.compile_only(work["duration"])
+            from cubie_amd.runtime import kcache
+            spec = k.spec_for(work["duration"], False, adjoint=True)
+            kcache.get_code(spec.source(), spec.compile_options())
+            continue
+        inits = torch.from_numpy(work["inits"]).cuda()
+        params = torch.from_numpy(work["params"]).cuda()
+        target = torch.from_numpy(work["target"]).cuda()
+        dur = work["duration"]
+        for _ in range(args.warmup):
+            k.run(inits, params, duration=dur, timed=True)
+        fwd = [k.run(inits, params, duration=dur, timed=True)[1] for _ in range(args.steps)]
+        rec = adj.record(k, inits, params, dur)           # counting pass sizes the tape
+        max_steps = rec.max_steps
+        del rec
+
+        def one(timed):
+            rec = adj.record(k, inits, params, dur, max_steps=max_steps, timed=timed)
+            summ = rec.outputs.state_summaries
+            g = 2.0 * (summ - target)
+            gi, gp, bms = adj.backward(rec, None, g, timed=timed)
+            status = rec.outputs.status_codes
+            res = (rec.record_ms, bms, gi, gp, status, summ)
+            del rec
+            return res
+
+        for _ in range(args.warmup):
+            one(False)
+        torch.cuda.synchronize()
+        recs, bwds = [], []
+        for _ in range(args.steps):
+            r_ms, b_ms, gi, gp, status, summ = one(True)
+            recs.append(r_ms)
+            bwds.append(b_ms)
+            print(f"# adjoint {alg}: record {r_ms:.1f} ms, backward {b_ms:.1f} ms, "
+                  f"forward-only {np.mean(fwd):.1f} ms", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        failed = int((status != 0).sum().item())
+        # central differences of the same FP64 forward solve on a few systems
+        m_fd = 4
+        X = np.concatenate([work["inits"][:, :m_fd], work["params"][:, :m_fd]])
+        nin = X.shape[0]
+        Xb = np.repeat(X[:, :, None, None], nin, axis=2).repeat(2, axis=3)
+        hstep = 1e-6 * np.maximum(np.abs(X), 1e-3)
+        for q in range(nin):
+            Xb[q, :, q, 0] += hstep[q]
+            Xb[q, :, q, 1] -= hstep[q]
+        Xb = Xb.reshape(nin, -1)
+        res = solver.solve(Xb[:4], Xb[4:], duration=dur, nan_error_trajectories=False)
+        tg = np.repeat(work["target"][:m_fd], nin * 2)
+        L = ((res.state_summaries[0, 0, :] - tg) ** 2).reshape(m_fd, nin, 2)
+        fd = ((L[:, :, 0] - L[:, :, 1]) / (2 * hstep.T)).T
+        grad = np.concatenate([gi.cpu().numpy()[:, :m_fd], gp.cpu().numpy()[:, :m_fd]])
+        fd_err = float((np.abs(grad - fd).max(axis=0) / np.abs(fd).max(axis=0)).max())
+        spec = k.spec_for(dur, False, adjoint=True)
+        spec.source()
+        ic = None
+        rec_ms, bwd_ms, fwd_ms = float(np.mean(recs)), float(np.mean(bwds)), float(np.mean(fwd))
+        out[alg] = {"value": n_sys / ((rec_ms + bwd_ms) * 1e-3), "record_ms": rec_ms,
+                    "backward_ms": bwd_ms, "forward_only_ms": fwd_ms,
+                    "adjoint_over_forward": (rec_ms + bwd_ms) / fwd_ms,
+                    "max_accepted_steps": max_steps, "failed_runs": failed,
+                    "fd_check": {"systems": m_fd, "inputs": nin, "relative_step": 1e-6,
+                                 "max_rel_err": fd_err},
+                    "backward_flops_per_step": adjoint_flops_per_step(spec),
+                    "kernel_info": k.module(spec, "adjoint").info()}
+    if args.compile_only:
+        return
+    head = names[0]
+    r = out[head]
+    work = hh_adjoint_workload(n_sys, algorithm=head, **kw)
+    # roofline of the backward kernel: its flops over all replayed steps
+    # (every system's accepted steps, bounded by the tape length)
+    line = {
+        "metric": "adjoint gradients/s", "value": r["value"], "unit": "gradients/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["record_ms"] + r["backward_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(work["config"], headline=head, variants=out),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        orc.build()
+        base = make_solver(work, np.float64, "plain", "plain")
+        run = orc.OracleRun(base.kernel, work["oracle_system"], work["duration"])
+        threads = os.cpu_count() or 1
+        sample = 16 * threads
+        t0 = time.perf_counter()
+        run.solve(work["inits"][:, :sample], work["params"][:, :sample], threads=threads)
+        el = time.perf_counter() - t0
+        if el < 4.0:       # grow once to a 10-30 s measurement
+            sample = int(min(n_sys, sample * max(2.0, 15.0 / max(el, 1e-3))))
+            t0 = time.perf_counter()
+            run.solve(work["inits"][:, :sample], work["params"][:, :sample], threads=threads)
+            el = time.perf_counter() - t0
+        per_grad = 2 * (work["inits"].shape[0] + work["params"].shape[0])
+        line["cpu_baseline"] = {
+            "value": sample / (el * per_grad), "unit": "gradients/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{sample} systems, FP64 oracle forward solves {el:.1f} s on {threads} "
+                      f"threads; central differences need {per_grad} solves per gradient"}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="hh", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="hh", choices=sorted(WORKLOADS) + ["hh_adjoint"])
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--duration", type=float, default=None)
     ap.add_argument("--variants", default=None,
@@ -364,6 +543,13 @@ def main():
                     help="instead of the benchmark, run a plain device copy of this many "
                          "bytes (the reference kernel for scaling HBM byte counters)")
     args = ap.parse_args()
+    args.n_given = any(a == "--n" or a.startswith("--n=") for a in sys.argv[1:])
+    if args.workload == "hh_adjoint":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only.", file=sys.stderr)
+            sys.exit(2)
+        adjoint_main(args)
+        return
     if args.copy_bytes:
         import torch
         src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
@@ -404,7 +590,15 @@ def main():
     head = next((nm for nm in names if results[nm]["failed_runs"] == 0), names[0])
     r = results[head]
     kernel = solvers[head].kernel
-    dtype = "f64" if kernel.precision is np.float64 else "f32"
+    spec_h = r["spec"]
+    if kernel.precision is np.float64:
+        dtype = "f64"
+    elif spec_h.mixed_implicit:
+        dtype = "f64+f32"     # FP64 Newton / residual / state, FP32 Krylov
+    elif spec_h.compensated:
+        dtype = "f32+f64"     # FP32 stages, compensated state, FP64 error norm
+    else:
+        dtype = "f32"
 
     # accuracy of every variant against the FP64 kernel's summaries
     ref = results.get("fp64_plain", {}).get("summaries")
@@ -418,10 +612,15 @@ def main():
             ok = (res["status"] == 0) & (results["fp64_plain"]["status"] == 0)
             a, b = res["summaries"][..., ok].astype(np.float64), ref[..., ok]
             scale = np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-300)
-            rel = (np.abs(a - b) / scale).max(axis=(0, 2)) if ok.any() else None
+            relall = (np.abs(a - b) / scale).max(axis=0) if ok.any() else None   # (cols, systems)
+            rel = relall.max(axis=1) if relall is not None else None
             names = work.get("summary_columns") or [str(i) for i in range(a.shape[1])]
             if rel is not None:
                 entry["rel_err_vs_fp64"] = {nm_: float(e) for nm_, e in zip(names, rel)}
+                entry["rel_err_vs_fp64_p99"] = {nm_: float(np.quantile(relall[i], 0.99))
+                                                for i, nm_ in enumerate(names)}
+                entry["rel_err_vs_fp64_median"] = {nm_: float(np.median(relall[i]))
+                                                   for i, nm_ in enumerate(names)}
                 entry["max_rel_err_vs_fp64"] = float(rel.max())
                 # integer-valued summaries: how many systems disagree at all
                 entry["systems_differing"] = {nm_: int((a[:, i] != b[:, i]).any(axis=0).sum())
@@ -471,8 +670,12 @@ def main():
         "config": dict(work["config"], headline=head, kernel_mode=kernel.kernel_mode,
                        precision_mode=kernel.precision_mode, attempted_steps=r["steps"],
                        accepted_steps=r["accepted"], variants=variants_out),
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[dtype], "unit": "TFLOP/s",
-                     "frac": (achieved / PEAK[dtype]) if achieved else None, "traffic": None,
+        # Mixed kernels are priced at the FP64 vector peak: their FP32 work
+        # is compiled without packed FP32 math, which issues at that rate.
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[dtype[:3]] if dtype == "f32" else PEAK["f64"],
+                     "unit": "TFLOP/s",
+                     "frac": (achieved / (PEAK[dtype[:3]] if dtype == "f32" else PEAK["f64"]))
+                     if achieved else None, "traffic": None,
                      "flops_per_attempted_step": fps},
     }
     if cpu is not None:

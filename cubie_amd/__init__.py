@@ -2,12 +2,16 @@
 
 Public entry points mirror the reference package (ccam80/cubie): ``solve_ivp``,
 ``Solver``, ``create_ODE_system`` / ``SymbolicODE``, ``ArrayInterpolator``,
-``summary_metrics`` / ``register_metric`` and ``CUBIE_RESULT_CODES``. The
+``summary_metrics`` / ``register_metric`` and ``CUBIE_RESULT_CODES``, plus
+the engine's own ``solve_adjoint`` (a differentiable batch solve whose
+backward pass is the discrete adjoint) and ``autotune``. The
 integration itself runs in HIP kernels compiled for gfx950 (see
 ``cubie_amd.csrc``); there is no CPU fallback.
 """
 
 from .array_interpolator import ArrayInterpolator
+from .batchsolving.adjoint import solve_adjoint
+from .batchsolving.autotune import autotune
 from .batchsolving.solver import Solver, solve_ivp
 from .batchsolving.solveresult import DeviceSolveResult, SolveResult
 from .integrators import tableaus
@@ -20,6 +24,6 @@ __all__ = [
     "ArrayInterpolator", "Solver", "solve_ivp", "SolveResult", "DeviceSolveResult",
     "SymbolicODE", "create_ODE_system", "summary_metrics", "register_metric",
     "SummaryMetric", "CUBIE_RESULT_CODES", "decode_status_codes", "tableaus",
-    "algorithm_is_adaptive", "resolve_algorithm",
+    "algorithm_is_adaptive", "resolve_algorithm", "solve_adjoint", "autotune",
 ]
 __version__ = "0.1.0"

@@ -1,0 +1,260 @@
+"""Discrete adjoint of a batch solve behind ``torch.autograd.Function``.
+
+For explicit Runge-Kutta (DOPRI5 and every other ERK tableau) and
+Rosenbrock-W solves, the gradient of a loss on the saved states and the
+state summaries (mean, rms, max, min) is computed with respect to every
+system's initial values and parameters by reverse-mode differentiation of
+the discrete solution map, with the accepted step sizes held at the values
+the forward pass chose.
+
+Two kernels from one module (``cubie_amd/csrc/device/adjoint.hpp``):
+
+* ``cubie_integrate_record`` integrates like ``cubie_integrate`` and
+  records, per system, every accepted step size, one flag byte per step
+  (save row / summary sample taken at its end) and a checkpoint of the state
+  every ``checkpoint_every`` accepted steps;
+* ``cubie_adjoint`` replays each checkpoint segment and runs the step
+  adjoints backwards, one lane per system.
+
+HBM per system: (sizeof(real) + 1) bytes per accepted step of tape,
+(n + 1) values per checkpoint, and replay buffers of
+(K + 1) (n + 1) + K s n values (K = checkpoint_every, s = stages). Tape length
+is the largest accepted-step count of the batch: when ``max_steps`` is not
+given a counting pass (the recording kernel with an empty tape) measures it
+first.
+
+The reference has no adjoint; this module is an extension of the engine.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Any, Optional
+
+import numpy as np
+
+from ..runtime import kcache, native
+from ..result_codes import CUBIE_RESULT_CODES
+from .kernel import CompileOnly, _compile_only
+
+# summary metrics whose gradient the backward pass propagates
+DIFFERENTIABLE_METRICS = {"mean", "rms", "max", "min"}
+
+
+@dataclass
+class AdjointRecord:
+    """Device state of one recorded forward solve."""
+    kernel: Any
+    spec: Any
+    setup: dict            # BatchSolverKernel.setup() result (inputs, outputs, args)
+    args: Any              # native.AdjointArgs
+    buffers: dict          # torch tensors of the tape and checkpoints
+    max_steps: int
+    max_checkpoints: int
+    checkpoint_every: int
+    record_ms: Optional[float] = None
+
+    @property
+    def outputs(self):
+        return self.setup["outputs"]
+
+
+def _grid(n_runs: int, block: int) -> int:
+    return max(1, math.ceil(n_runs / block))
+
+
+def record(kernel, inits, params, duration: float, warmup: float = 0.0, t0: float = 0.0,
+           rtol_per_run=None, duration_per_run=None, max_steps: Optional[int] = None,
+           checkpoint_every: int = 32, timed: bool = False) -> AdjointRecord:
+    """Forward solve that records the adjoint's tape; returns the record
+    (outputs under ``record.outputs``)."""
+    if checkpoint_every < 1:
+        raise ValueError("checkpoint_every must be at least 1")
+    if _compile_only():
+        _compile(kernel, duration, rtol_per_run is not None, checkpoint_every)
+    import torch
+    st = kernel.setup(inits, params, duration, warmup, t0, rtol_per_run, duration_per_run)
+    spec = kernel.spec_for(st["duration"], st["per_run_rtol"], adjoint=True,
+                           adj_ckpt_every=checkpoint_every)
+    mod = kernel.module(spec, "record")
+    dev = torch.device("cuda")
+    n_runs = st["n_runs"]
+    n = kernel.system.num_states
+    tdt = torch.float32 if kernel.precision is np.float32 else torch.float64
+    block = spec.block_size
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def launch(max_st: int, max_ck: int, timed_: bool):
+        bufs = {
+            "tape_dt": torch.empty((max(max_st, 1), n_runs), dtype=tdt, device=dev),
+            "tape_flags": torch.empty((max(max_st, 1), n_runs), dtype=torch.uint8, device=dev),
+            "checkpoints": torch.empty((max(max_ck, 1), n, n_runs), dtype=tdt, device=dev),
+            "checkpoint_t": torch.empty((max(max_ck, 1), n_runs), dtype=torch.float64, device=dev),
+            "tape_info": torch.zeros((3, n_runs), dtype=torch.int32, device=dev),
+        }
+        a = native.AdjointArgs()
+        a.fwd = st["args"]
+        a.checkpoints = bufs["checkpoints"].data_ptr()
+        a.checkpoint_t = bufs["checkpoint_t"].data_ptr()
+        a.tape_dt = bufs["tape_dt"].data_ptr()
+        a.tape_flags = bufs["tape_flags"].data_ptr()
+        a.tape_info = bufs["tape_info"].data_ptr()
+        a.max_steps = int(max_st)
+        a.max_checkpoints = int(max_ck)
+        ms = mod.launch_adjoint(a, _grid(n_runs, block), block, stream, timed=timed_)
+        return a, bufs, ms
+
+    if max_steps is None:
+        # counting pass: an empty tape, the kernel still counts the steps
+        _, cbufs, _ = launch(0, 0, False)
+        max_steps = int(cbufs["tape_info"][0].max().item()) if n_runs else 0
+        st["outputs"].status_codes.zero_()
+    max_steps = int(max_steps)
+    max_ck = max_steps // checkpoint_every + 1
+    args, bufs, ms = launch(max_steps, max_ck, timed)
+    return AdjointRecord(kernel=kernel, spec=spec, setup=st, args=args, buffers=bufs,
+                         max_steps=max_steps, max_checkpoints=max_ck,
+                         checkpoint_every=checkpoint_every, record_ms=ms)
+
+
+def _compile(kernel, duration, per_run_rtol, checkpoint_every):
+    """CUBIE_COMPILE_ONLY=1: cache the adjoint module and stop."""
+    spec = kernel.spec_for(float(np.float64(duration)), per_run_rtol, adjoint=True,
+                           adj_ckpt_every=checkpoint_every)
+    kcache.get_code(spec.source(), spec.compile_options())
+    raise CompileOnly(spec)
+
+
+def _check_summary_grads(kernel, grad_summ) -> None:
+    """Summary metrics other than mean / rms / max / min carry no gradient;
+    a nonzero gradient for one of them is an error rather than a silent 0."""
+    from ..outputhandling.summary_metrics import summary_metrics
+    oc = kernel.output_config
+    if grad_summ is None or not oc.summarise_state:
+        return
+    bad, off = [], 0
+    for nm, prm in oc.metrics:
+        h = summary_metrics[nm].output_len(prm)
+        if nm not in DIFFERENTIABLE_METRICS:
+            bad.append((nm, off, h))
+        off += h
+    if not bad:
+        return
+    oh = oc.summaries_output_height_per_var
+    g = grad_summ.reshape(grad_summ.shape[0], -1, oh, grad_summ.shape[-1])
+    for nm, off, h in bad:
+        if bool((g[:, :, off:off + h, :] != 0).any()):
+            raise NotImplementedError(
+                f"summary metric {nm!r} is not differentiable by the adjoint; "
+                f"differentiable metrics: {sorted(DIFFERENTIABLE_METRICS)}")
+
+
+def backward(rec: AdjointRecord, grad_state=None, grad_state_summaries=None,
+             timed: bool = False):
+    """Gradients of the loss with respect to inits and params, given the
+    loss's gradients with respect to the forward outputs (state rows and
+    state summaries, each laid out like that output, or None)."""
+    import torch
+    kernel, spec, st = rec.kernel, rec.spec, rec.setup
+    status = st["outputs"].status_codes
+    full = int(CUBIE_RESULT_CODES.ADJOINT_TAPE_FULL)
+    if bool(((status & full) != 0).any()):
+        raise RuntimeError("the adjoint tape overflowed for some runs; record with a larger "
+                           "max_steps")
+    dev = torch.device("cuda")
+    n_runs = st["n_runs"]
+    n = kernel.system.num_states
+    npar = kernel.system.num_parameters
+    tdt = torch.float32 if kernel.precision is np.float32 else torch.float64
+    K = rec.checkpoint_every
+    S = spec.adjoint_stash
+    outs = st["outputs"]
+
+    def grad_like(g, ref):
+        if g is None:
+            return None
+        g = g.to(device=dev, dtype=tdt).contiguous()
+        if tuple(g.shape) != tuple(ref.shape):
+            raise ValueError(f"gradient shape {tuple(g.shape)} != output shape {tuple(ref.shape)}")
+        return g
+
+    gs = grad_like(grad_state, outs.state)
+    gss = grad_like(grad_state_summaries, outs.state_summaries)
+    _check_summary_grads(kernel, gss)
+    bufs = {
+        "replay": torch.empty((K + 1, n, n_runs), dtype=tdt, device=dev),
+        "replay_t": torch.empty((K + 1, n_runs), dtype=torch.float64, device=dev),
+        "replay_stages": torch.empty((K, S, n_runs), dtype=tdt, device=dev),
+        "grad_inits": torch.zeros((n, n_runs), dtype=tdt, device=dev),
+        "grad_params": torch.zeros((max(npar, 1), n_runs), dtype=tdt, device=dev),
+    }
+    a = native.AdjointArgs()
+    a.fwd = st["args"]
+    for k in ("checkpoints", "checkpoint_t", "tape_dt", "tape_flags", "tape_info"):
+        setattr(a, k, getattr(rec.args, k))
+    a.replay = bufs["replay"].data_ptr()
+    a.replay_t = bufs["replay_t"].data_ptr()
+    a.replay_stages = bufs["replay_stages"].data_ptr()
+    a.grad_state = gs.data_ptr() if gs is not None else 0
+    a.grad_state_summ = gss.data_ptr() if gss is not None else 0
+    a.grad_inits = bufs["grad_inits"].data_ptr()
+    a.grad_params = bufs["grad_params"].data_ptr()
+    a.max_steps = rec.max_steps
+    a.max_checkpoints = rec.max_checkpoints
+    mod = kernel.module(spec, "adjoint")
+    stream = torch.cuda.current_stream().cuda_stream
+    ms = mod.launch_adjoint(a, _grid(n_runs, spec.block_size), spec.block_size, stream,
+                            timed=timed)
+    gp = bufs["grad_params"][:npar]
+    return bufs["grad_inits"], gp, ms
+
+
+def _function():
+    import torch
+
+    class SolveAdjointFunction(torch.autograd.Function):
+        """(inits, params) -> (state rows, state summaries, status); the
+        backward pass is the discrete adjoint."""
+
+        @staticmethod
+        def forward(ctx, inits, params, kernel, duration, opts):
+            rec = record(kernel, inits.detach(), params.detach(), duration, **opts)
+            ctx.rec = rec
+            out = rec.outputs
+            ctx.mark_non_differentiable(out.status_codes)
+            return out.state, out.state_summaries, out.status_codes
+
+        @staticmethod
+        def backward(ctx, g_state, g_summ, _g_status):
+            gi, gp = backward(ctx.rec, g_state, g_summ)[:2]
+            return gi, gp, None, None, None
+
+    return SolveAdjointFunction
+
+
+_FN = None
+
+
+def solve_adjoint(solver, inits, params, duration: float, warmup: float = 0.0, t0: float = 0.0,
+                  max_steps: Optional[int] = None, checkpoint_every: int = 32,
+                  rtol_per_run=None, duration_per_run=None):
+    """Differentiable batch solve: returns torch tensors ``(state,
+    state_summaries, status)`` laid out like the solver's device outputs
+    (``("time", "variable", "run")``). ``inits`` (n_states, n_runs) and
+    ``params`` (n_params, n_runs) may require grad; ``loss.backward()`` then
+    runs the discrete adjoint."""
+    global _FN
+    kernel = getattr(solver, "kernel", solver)
+    if _compile_only():
+        _compile(kernel, duration, rtol_per_run is not None, checkpoint_every)
+    import torch
+    if _FN is None:
+        _FN = _function()
+    prec = torch.float32 if kernel.precision is np.float32 else torch.float64
+    dev = torch.device("cuda")
+    inits = torch.as_tensor(inits, dtype=prec).to(dev)
+    params = torch.as_tensor(params, dtype=prec).to(dev)
+    opts = dict(warmup=warmup, t0=t0, max_steps=max_steps, checkpoint_every=checkpoint_every,
+                rtol_per_run=rtol_per_run, duration_per_run=duration_per_run)
+    return _FN.apply(inits, params, kernel, float(duration), opts)

@@ -397,7 +397,7 @@ __device__ __forceinline__ void add_output_grads(const cubie_adjoint_args& A, lo
         if constexpr (Cfg::save_state) {
             constexpr int W = Cfg::n_saved_states + (Cfg::save_time ? 1 : 0);
             const io_real* g = static_cast<const io_real*>(A.grad_state);
-            if (g != nullptr && save_i < a.out_rows) {
+            if (g != nullptr && save_i >= 0 && save_i < a.out_rows) {
 #pragma unroll
                 for (int k = 0; k < Cfg::n_saved_states; ++k)
                     lam[Cfg::saved_state_idx[k]] += real(g[(save_i * W + k) * N + run]);
@@ -412,7 +412,7 @@ __device__ __forceinline__ void add_output_grads(const cubie_adjoint_args& A, lo
             const long long row = upd_i / Cfg::samples_per_summary;
             const io_real* g = static_cast<const io_real*>(A.grad_state_summ);
             const io_real* out = static_cast<const io_real*>(a.state_summ_out);
-            if (g != nullptr && row < a.summary_rows) {
+            if (g != nullptr && upd_i >= 0 && row < a.summary_rows) {
                 const real inv_spp = real(1.0 / double(Cfg::samples_per_summary));
 #pragma unroll
                 for (int v = 0; v < NS; ++v) {

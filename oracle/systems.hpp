@@ -23,7 +23,9 @@ enum SystemId {
     SYS_LINEAR3 = 5,       // params p0..p2, constants c0..c2, driver d0, obs o0..o2
     SYS_CALLBACK = 6,      // Python callbacks
     SYS_DECAY = 7,         // dy_i = -k_i y_i, params k_i (closed form)
+    SYS_HODGKIN_HUXLEY6 = 8,  // params gNa, gK, gL, ENa, EK, EL; driver I_ext
 };
+
 
 // Callback signatures used by SYS_CALLBACK (always double precision).
 typedef void (*rhs_cb_t)(const double* y, const double* p, const double* d,
@@ -51,6 +53,15 @@ static const double HH_ENA = 50.0;
 static const double HH_EK = -77.0;
 static const double HH_EL = -54.387;
 static const double HH_CM = 1.0;
+
+// Reversal potentials: constants, or parameters 3..5 of the six-parameter HH.
+template <typename T>
+inline void hh_reversals(const SystemDesc& s, const T* p, T& ena, T& ek, T& el) {
+    const bool six = s.id == SYS_HODGKIN_HUXLEY6;
+    ena = six ? p[3] : T(HH_ENA);
+    ek = six ? p[4] : T(HH_EK);
+    el = six ? p[5] : T(HH_EL);
+}
 
 // x / (1 - exp(-x/10)) has a removable singularity at x = 0 (value 10); within
 // |x| < 1e-3 the rates use its first-order expansion 10 + x/2, so a stage state
@@ -130,13 +141,15 @@ void sys_rhs(const SystemDesc& s, const T* y, const T* p, const T* d, T* obs,
         dy[2] = r2;
         break;
     }
-    case SYS_HODGKIN_HUXLEY: {
-        T am, bm, ah, bh, an, bn;
+    case SYS_HODGKIN_HUXLEY:
+    case SYS_HODGKIN_HUXLEY6: {
+        T am, bm, ah, bh, an, bn, ena, ek, el;
         const T V = y[0], m = y[1], h = y[2], n = y[3];
         hh_rates<T>(V, am, bm, ah, bh, an, bn);
-        const T INa = p[0] * m * m * m * h * (V - T(HH_ENA));
-        const T IK = p[1] * n * n * n * n * (V - T(HH_EK));
-        const T IL = p[2] * (V - T(HH_EL));
+        hh_reversals<T>(s, p, ena, ek, el);
+        const T INa = p[0] * m * m * m * h * (V - ena);
+        const T IK = p[1] * n * n * n * n * (V - ek);
+        const T IL = p[2] * (V - el);
         dy[0] = (d[0] - INa - IK - IL) / T(HH_CM);
         dy[1] = am * (T(1) - m) - bm * m;
         dy[2] = ah * (T(1) - h) - bh * h;
@@ -197,11 +210,13 @@ void sys_jvp(const SystemDesc& s, const T* y, const T* p, const T* d,
         jv[2] = T(2) * k2 * y[1] * v[1];
         break;
     }
-    case SYS_HODGKIN_HUXLEY: {
+    case SYS_HODGKIN_HUXLEY:
+    case SYS_HODGKIN_HUXLEY6: {
         // Central-free analytic derivative of each rate with respect to V.
         const T V = y[0], m = y[1], h = y[2], n = y[3];
-        T am, bm, ah, bh, an, bn;
+        T am, bm, ah, bh, an, bn, ena, ek, el;
         hh_rates<T>(V, am, bm, ah, bh, an, bn);
+        hh_reversals<T>(s, p, ena, ek, el);
         const T dam = T(0.1) * hh_dxratio(V + T(40));
         const T dan = T(0.01) * hh_dxratio(V + T(55));
         const T dbm = -bm / T(18);
@@ -212,9 +227,9 @@ void sys_jvp(const SystemDesc& s, const T* y, const T* p, const T* d,
         const T gna = p[0], gk = p[1], gl = p[2];
         const T m3 = m * m * m, n4 = n * n * n * n;
         const T dVdV = -(gna * m3 * h + gk * n4 + gl) / T(HH_CM);
-        const T dVdm = -(gna * T(3) * m * m * h * (V - T(HH_ENA))) / T(HH_CM);
-        const T dVdh = -(gna * m3 * (V - T(HH_ENA))) / T(HH_CM);
-        const T dVdn = -(gk * T(4) * n * n * n * (V - T(HH_EK))) / T(HH_CM);
+        const T dVdm = -(gna * T(3) * m * m * h * (V - ena)) / T(HH_CM);
+        const T dVdh = -(gna * m3 * (V - ena)) / T(HH_CM);
+        const T dVdn = -(gk * T(4) * n * n * n * (V - ek)) / T(HH_CM);
         jv[0] = dVdV * v[0] + dVdm * v[1] + dVdh * v[2] + dVdn * v[3];
         jv[1] = (dam * (T(1) - m) - dbm * m) * v[0] - (am + bm) * v[1];
         jv[2] = (dah * (T(1) - h) - dbh * h) * v[0] - (ah + bh) * v[2];
@@ -256,6 +271,7 @@ void sys_dfdt(const SystemDesc& s, const T* y, const T* p, const T* d,
     for (int i = 0; i < s.n; ++i) out[i] = T(0);
     switch (s.id) {
     case SYS_HODGKIN_HUXLEY:
+    case SYS_HODGKIN_HUXLEY6:
         out[0] = ddt[0] / T(HH_CM);
         break;
     case SYS_CALLBACK: {

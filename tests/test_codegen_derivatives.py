@@ -1,0 +1,132 @@
+"""The generated system functions, compiled for the host with g++ from the
+same HIP C++ text the device compiles, against finite differences of the
+generated right-hand side (no GPU): the dense Jacobian, the parameter
+Jacobian and the discrete adjoint's building blocks vjp (J^T w, f_p^T w),
+hvp (gradients of z . J u) and ftvjp (gradients of z . df/dt)."""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import bench
+from cubie_amd.odesystems.codegen import SystemCodegen
+from systems import hodgkin_huxley, lorenz, robertson, van_der_pol
+
+PRE = """#include <cmath>
+#define __device__
+#define __forceinline__ inline
+using std::exp; using std::fabs; using std::sin; using std::cos; using std::log; using std::sqrt;
+"""
+POST = """
+extern "C" {
+void c_dxdt(const double* y, const double* p, const double* d, double* dy, double t) {
+  double obs[64]; System::dxdt<double>(y, p, d, obs, dy, t); }
+void c_jac(const double* y, const double* p, const double* d, double* J, double t) {
+  System::jac<double>(y, p, d, J, t); }
+void c_jacp(const double* y, const double* p, const double* d, double* J, double t) {
+  System::jacp<double>(y, p, d, J, t); }
+void c_dfdt(const double* y, const double* p, const double* d, const double* dd, double* o, double t) {
+  System::dfdt<double>(y, p, d, dd, o, t); }
+void c_vjp(const double* y, const double* p, const double* d, const double* w, double* gy, double* gp, double t) {
+  System::vjp<double>(y, p, d, w, gy, gp, t); }
+void c_hvp(const double* y, const double* p, const double* d, const double* u, const double* z, double* gy, double* gp, double t) {
+  System::hvp<double>(y, p, d, u, z, gy, gp, t); }
+void c_ftvjp(const double* y, const double* p, const double* d, const double* dd, const double* z, double* gy, double* gp, double t) {
+  System::ftvjp<double>(y, p, d, dd, z, gy, gp, t); }
+}
+"""
+
+
+class HostSystem:
+    def __init__(self, system, tmp_path):
+        src = PRE + SystemCodegen(system, np.float64, adjoint=True).emit() + POST
+        cpp = os.path.join(tmp_path, f"{system.name}.cpp")
+        so = os.path.join(tmp_path, f"lib{system.name}.so")
+        open(cpp, "w").write(src)
+        subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", "-o", so, cpp], check=True)
+        self.lib = ctypes.CDLL(so)
+        self.n, self.p, self.nd = system.num_states, system.num_parameters, system.num_drivers
+
+    @staticmethod
+    def _p(a):
+        return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def call(self, name, *arrays, outs, t=0.3):
+        bufs = [np.zeros(k) for k in outs]
+        args = [self._p(a) for a in arrays] + [b.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for b in bufs]
+        getattr(self.lib, name)(*args, ctypes.c_double(t))
+        return bufs
+
+    def f(self, y, p, d):
+        return self.call("c_dxdt", y, p, d, outs=[self.n])[0]
+
+
+def _fd(fun, x, rel=1e-6):
+    g = []
+    for k in range(x.size):
+        e = np.zeros_like(x)
+        e[k] = rel * max(abs(x[k]), 1.0)
+        g.append((fun(x + e) - fun(x - e)) / (2 * e[k]))
+    return np.array(g)          # (len(x), ...) derivative along axis 0
+
+
+CASES = [
+    (lambda: hodgkin_huxley(np.float64), [-52.0, 0.21, 0.45, 0.41], [118.0, 37.0, 0.31], [6.5]),
+    (lambda: bench.hh6_system(np.float64), [-39.9995, 0.3, 0.4, 0.5],
+     [110.0, 33.0, 0.28, 52.0, -75.0, -55.0], [-3.0]),
+    (lambda: lorenz(np.float64), [1.3, -0.7, 24.0], [10.0, 28.0, 8.0 / 3.0], []),
+    (lambda: robertson(np.float64), [0.9, 2e-5, 0.1], [0.04, 3e7, 1e4], []),
+    (lambda: van_der_pol(np.float64), [1.5, -0.3], [7.0], []),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_generated_derivatives_match_finite_differences(case, tmp_path):
+    make, y, p, d = CASES[case]
+    system = make()
+    hs = HostSystem(system, str(tmp_path))
+    y, p, d = (np.array(v, dtype=float) for v in (y, p, d if d else [0.0]))
+    n, npar = hs.n, hs.p
+    rng = np.random.default_rng(case)
+    f0 = hs.f(y, p, d)
+    scale = max(np.abs(f0).max(), 1.0)
+
+    J = hs.call("c_jac", y, p, d, outs=[n * n])[0].reshape(n, n)
+    Jfd = _fd(lambda yy: hs.f(yy, p, d), y).T
+    assert np.abs(J - Jfd).max() <= 1e-6 * max(np.abs(Jfd).max(), 1.0)
+    if npar:
+        Jp = hs.call("c_jacp", y, p, d, outs=[n * npar])[0].reshape(n, npar)
+        Jpfd = _fd(lambda pp: hs.f(y, pp, d), p).T
+        assert np.abs(Jp - Jpfd).max() <= 1e-6 * max(np.abs(Jpfd).max(), 1.0)
+
+    w = rng.normal(size=n)
+    gy, gp = hs.call("c_vjp", y, p, d, w, outs=[n, max(npar, 1)])
+    assert np.allclose(gy, Jfd.T @ w, rtol=1e-6, atol=1e-6 * scale * np.abs(w).max())
+    if npar:
+        assert np.allclose(gp[:npar], Jpfd.T @ w, rtol=1e-6,
+                           atol=1e-6 * max(np.abs(Jpfd).max(), 1.0) * np.abs(w).max())
+
+    u, z = rng.normal(size=n), rng.normal(size=n)
+
+    def zju(yy, pp):
+        Jm = hs.call("c_jac", yy, pp, d, outs=[n * n])[0].reshape(n, n)
+        return z @ Jm @ u
+
+    hy, hp = hs.call("c_hvp", y, p, d, u, z, outs=[n, max(npar, 1)])
+    hy_fd = _fd(lambda yy: np.array(zju(yy, p)), y)
+    assert np.allclose(hy, hy_fd, rtol=1e-5, atol=1e-6 * max(np.abs(hy_fd).max(), 1.0))
+    if npar:
+        hp_fd = _fd(lambda pp: np.array(zju(y, pp)), p)
+        assert np.allclose(hp[:npar], hp_fd, rtol=1e-5, atol=1e-6 * max(np.abs(hp_fd).max(), 1.0))
+
+    dd = rng.normal(size=max(hs.nd, 1))
+    ty, tp = hs.call("c_ftvjp", y, p, d, dd, z, outs=[n, max(npar, 1)])
+
+    def zft(yy, pp):
+        return z @ hs.call("c_dfdt", yy, pp, d, dd, outs=[n])[0]
+
+    ty_fd = _fd(lambda yy: np.array(zft(yy, p)), y)
+    assert np.allclose(ty, ty_fd, rtol=1e-5, atol=1e-8)
