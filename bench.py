@@ -200,7 +200,41 @@ def vdp_workload(n, precision=np.float64, duration=100.0):
                        "n_systems": n}}
 
 
-WORKLOADS = {"lorenz": lorenz_workload, "hh": hh_workload, "vdp": vdp_workload}
+def robertson_system(precision):
+    import cubie_amd as ca
+    return ca.create_ODE_system(
+        ["dy1 = -k1*y1 + k3*y2*y3", "dy2 = k1*y1 - k2*y2**2 - k3*y2*y3", "dy3 = k2*y2**2"],
+        states={"y1": 1.0, "y2": 0.0, "y3": 0.0},
+        parameters={"k1": 0.04, "k2": 3e7, "k3": 1e4}, precision=precision, name="robertson")
+
+
+def robertson_workload(n, precision=np.float64, duration=1e5, algorithm="ros3p"):
+    """Robertson kinetics batch: rate constants 0.04 / 3e7 / 1e4 each scaled
+    by exp(U[ln 0.8, ln 1.25]); y0 = (1, 0, 0); t in [0, 1e5] with 200
+    log-spaced save times (1e-5 ... 1e5) from an explicit save table; rtol
+    1e-6, atol 1e-10, BiCGSTAB."""
+    rng = np.random.default_rng(5)
+    f32 = lambda x: np.asarray(x, dtype=np.float32).astype(np.float64)
+    params = f32(np.array([0.04, 3e7, 1e4])[:, None]
+                 * np.exp(rng.uniform(np.log(0.8), np.log(1.25), (3, n))))
+    inits = np.tile(np.array([[1.0], [0.0], [0.0]]), (1, n))
+    table = np.logspace(-5.0, np.log10(duration), 200)
+    table[-1] = duration
+    solver_kw = dict(algorithm=algorithm, atol=1e-10, rtol=1e-6,
+                     linear_correction_type="bicgstab", save_times=table,
+                     output_types=["state", "iteration_counters"])
+    return {"name": f"robertson_{algorithm}", "system_factory": robertson_system,
+            "inits": inits, "params": params, "duration": float(duration),
+            "solver_kw": solver_kw, "oracle_system": "robertson",
+            "config": {"workload": f"Robertson kinetics, {algorithm} + BiCGSTAB, rtol 1e-6 "
+                                   f"atol 1e-10, t in [0,{duration:g}], 200 log-spaced saves",
+                       "n_systems": n}}
+
+
+WORKLOADS = {"lorenz": lorenz_workload, "hh": hh_workload, "vdp": vdp_workload,
+             "robertson": robertson_workload,
+             "robertson_radau": lambda n, **kw: robertson_workload(n, algorithm="radau", **kw)}
+DEFAULT_N = {"robertson": 65536, "robertson_radau": 65536}
 
 
 # ---------------------------------------------------------------------------
@@ -304,7 +338,10 @@ VARIANTS = {
     "fp64_persistent": (np.float64, "persistent", "plain"),
 }
 DEFAULT_VARIANTS = {"hh": ["fp32_compensated_persistent", "fp64_plain"], "lorenz": ["fp64_plain"],
-                    "vdp": ["fp32_compensated_persistent", "fp64_persistent", "fp64_plain"]}
+                    "vdp": ["fp32_compensated_persistent", "fp64_persistent", "fp64_plain"],
+                    "robertson": ["fp64_plain", "fp32_plain", "fp32_compensated_persistent"],
+                    "robertson_radau": ["fp64_plain", "fp32_plain",
+                                        "fp32_compensated_persistent"]}
 
 
 def tune_space(kernel_mode):
@@ -566,7 +603,8 @@ def main():
         sys.exit(2)
 
     kw = {} if args.duration is None else {"duration": args.duration}
-    work = WORKLOADS[args.workload](args.n, **kw)
+    n_sys = args.n if args.n_given else DEFAULT_N.get(args.workload, args.n)
+    work = WORKLOADS[args.workload](n_sys, **kw)
     names = (args.variants.split(",") if args.variants
              else DEFAULT_VARIANTS.get(args.workload, ["fp64_plain"]))
     solvers = {nm: make_solver(work, *VARIANTS[nm]) for nm in names}

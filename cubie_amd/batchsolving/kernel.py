@@ -131,7 +131,8 @@ class BatchSolverKernel:
         self.timing = LoopTiming.resolve(
             self.precision, ls.get("save_every"), ls.get("summarise_every"),
             ls.get("sample_summaries_every") or self._output_settings.get("sample_summaries_every"),
-            self.output_config.time_domain_outputs, self.output_config.save_summaries)
+            self.output_config.time_domain_outputs, self.output_config.save_summaries,
+            save_times=ls.get("save_times"))
         self._spec_cache: Dict[Any, KernelSpec] = {}
 
     def _implicit_settings(self) -> ImplicitSettings:
@@ -215,6 +216,10 @@ class BatchSolverKernel:
     def _validate_timing(self, timing: LoopTiming, duration: float) -> None:
         end_time = float(self.precision(duration)) + float(self.precision(self.controller.dt_min))
         oc = self.output_config
+        if oc.time_domain_outputs and timing.save_table:
+            if timing.save_times[-1] > float(duration) * (1.0 + 1e-12):
+                raise ValueError(f"the last save time ({timing.save_times[-1]}) is past the "
+                                 f"duration ({duration})")
         if oc.time_domain_outputs and timing.save_every is not None:
             if timing.save_every_p > end_time and not timing.save_last:
                 raise ValueError(f"save_every ({timing.save_every}) > duration ({duration}) so "
@@ -382,6 +387,8 @@ class BatchSolverKernel:
         dur_d = (torch.from_numpy(np.asarray(duration_per_run, dtype=np.float64)).to(dev)
                  if duration_per_run is not None else None)
         counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        table_d = (torch.tensor(timing.save_times, dtype=torch.float64, device=dev)
+                   if timing.save_table else None)
 
         args = native.SolveArgs()
         args.inits = d_inits.data_ptr()
@@ -395,7 +402,7 @@ class BatchSolverKernel:
         args.status_out = status.data_ptr()
         args.rtol_per_run = rtol_d.data_ptr() if rtol_d is not None else 0
         args.duration_per_run = dur_d.data_ptr() if dur_d is not None else 0
-        args.save_times = 0
+        args.save_times = table_d.data_ptr() if table_d is not None else 0
         args.work_counter = counter.data_ptr()
         args.checkpoints = 0
         args.checkpoint_stride = 0
@@ -415,4 +422,4 @@ class BatchSolverKernel:
         return {"args": args, "outputs": dout, "shapes": shapes, "n_runs": n_runs,
                 "duration": duration, "per_run_rtol": per_run_rtol, "params": d_params,
                 "inits": d_inits,
-                "keepalive": (d_inits, d_params, coeffs, rtol_d, dur_d, counter)}
+                "keepalive": (d_inits, d_params, coeffs, rtol_d, dur_d, counter, table_d)}

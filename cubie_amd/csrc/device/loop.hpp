@@ -105,7 +105,7 @@ struct Integrator {
     Vec<real, Cfg::n_error> error;
     long long run;
     double t;
-    real t_prec, t_end, save_stop, summary_stop, next_save, next_update, dt, dt_raw;
+    real t_prec, t_end, save_stop, summary_stop, next_save, next_update, dt, dt_raw, save_start;
     nreal rtol_run;
     int counters_since_save[4];
     int proposed_counters[2];
@@ -164,8 +164,10 @@ struct Integrator {
             Sys::observables(state.data(), ctx.params.data(), ctx.drivers.data(),
                              ctx.observables.data(), t_prec);
         if constexpr (Record) tape.checkpoint(0, state, t);
+        save_start = next_save;
         if (settling == real(0)) {
-            if constexpr (Cfg::save_regularly) next_save = real(next_save + Cfg::save_every);
+            if constexpr (Cfg::save_table) next_save = table_save(a, 1);
+            else if constexpr (Cfg::save_regularly) next_save = real(next_save + Cfg::save_every);
             if constexpr (Cfg::summarise_regularly) next_update = real(Cfg::sample_summaries_every + next_update);
             save_state<Cfg>(a, run, save_idx, state, ctx.observables, counters_since_save, t_prec);
             save_idx += 1;
@@ -176,6 +178,14 @@ struct Integrator {
         dt = Cfg::dt0;
         dt_raw = dt;
         irrecoverable = false;
+    }
+
+    // Time of save row k >= 1 from the explicit table (row 0 is the start);
+    // past the table's end the schedule is exhausted (+inf).
+    __device__ __forceinline__ real table_save(const cubie_solve_args& a, long long k) const {
+        const long long n_table = a.out_rows - 1;
+        return (k - 1 < n_table) ? real(double(save_start) + a.save_times[k - 1])
+                                 : real(__builtin_huge_val());
     }
 
     // One attempted step. Returns true once the system is finished.
@@ -287,7 +297,8 @@ struct Integrator {
             if (accept) tape.step(dt_eff, do_save, do_update, state, t);
         }
         if (do_save) {
-            if constexpr (Cfg::save_regularly) next_save += Cfg::save_every;
+            if constexpr (Cfg::save_table) next_save = table_save(a, save_idx + 1);
+            else if constexpr (Cfg::save_regularly) next_save += Cfg::save_every;
             save_state<Cfg>(a, run, save_idx, state, ctx.observables, counters_since_save, t_prec);
             save_idx += 1;
             if constexpr (Cfg::save_counters) {

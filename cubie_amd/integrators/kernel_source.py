@@ -207,6 +207,7 @@ class KernelSpec:
         a(f"  static constexpr int samples_per_summary = {max(tm.samples_per_summary, 1) if out.save_summaries else 1};")
         a(f"  static constexpr bool save_last = {str(tm.save_last).lower()};")
         a(f"  static constexpr bool save_regularly = {str(tm.save_regularly).lower()};")
+        a(f"  static constexpr bool save_table = {str(tm.save_table).lower()};")
         a(f"  static constexpr bool summarise_regularly = {str(tm.summarise_regularly).lower()};")
         # outputs
         a(f"  static constexpr bool save_state = {str(out.save_state).lower()};")

@@ -30,13 +30,28 @@ class LoopTiming:
     save_regularly: bool = False
     summarise_regularly: bool = False
     duration_dependent: bool = False
+    # Explicit save times (offsets from t0 + settling, increasing), in place
+    # of save_every: row 0 at the start, row k at save_times[k-1]. An engine
+    # extension (the reference saves on a uniform grid only).
+    save_times: Optional[tuple] = None
+
+    @property
+    def save_table(self) -> bool:
+        return self.save_times is not None
 
     @classmethod
     def resolve(cls, precision, save_every, summarise_every, sample_summaries_every,
-                has_time_domain: bool, has_summaries: bool) -> "LoopTiming":
+                has_time_domain: bool, has_summaries: bool, save_times=None) -> "LoopTiming":
         save_last = False
         duration_dependent = False
-        if has_time_domain and save_every is None:
+        if save_times is not None:
+            if save_every is not None:
+                raise ValueError("save_every and save_times are mutually exclusive")
+            st = np.asarray(save_times, dtype=np.float64).reshape(-1)
+            if st.size == 0 or not np.all(np.isfinite(st)) or st[0] <= 0 or np.any(np.diff(st) <= 0):
+                raise ValueError("save_times must be positive, finite and strictly increasing")
+            save_times = tuple(float(v) for v in st)
+        if has_time_domain and save_every is None and save_times is None:
             save_last = True
         if has_summaries:
             if summarise_every is None:
@@ -48,9 +63,10 @@ class LoopTiming:
             sample_summaries_every = None
         t = cls(precision=precision, save_every=save_every, summarise_every=summarise_every,
                 sample_summaries_every=sample_summaries_every, save_last=save_last,
-                save_regularly=save_every is not None and has_time_domain,
+                save_regularly=(save_every is not None or save_times is not None) and has_time_domain,
                 summarise_regularly=summarise_every is not None and has_summaries,
-                duration_dependent=duration_dependent)
+                duration_dependent=duration_dependent,
+                save_times=save_times if has_time_domain else None)
         if duration_dependent:
             warnings.warn(
                 "Summary metrics were requested with no summarise_every or "
@@ -66,7 +82,7 @@ class LoopTiming:
             return self
         return LoopTiming(self.precision, self.save_every, float(duration),
                           float(duration) / 100, self.save_last, self.save_regularly,
-                          True, True)
+                          True, True, self.save_times)
 
     # precision-cast views (ODELoopConfig properties)
     def _p(self, v):
@@ -112,7 +128,20 @@ class LoopTiming:
         allowance = min(4.0 * float(np.finfo(p).eps) * float(total), 0.49)
         return int(np.floor(total + allowance))
 
+    def save_table_starts(self, settling_time: float, t0: float) -> float:
+        """Start of the save schedule in the working precision."""
+        p = self.precision
+        return float(p(p(settling_time) + p(t0)))
+
+    def save_table_time(self, k: int, settling_time: float, t0: float) -> float:
+        """Time of save row k >= 1 as the device forms it:
+        real(double(start) + save_times[k - 1])."""
+        start = np.float64(self.save_table_starts(settling_time, t0))
+        return float(self.precision(start + np.float64(self.save_times[k - 1])))
+
     def output_length(self, duration: float) -> int:
+        if self.save_table:
+            return 1 + len(self.save_times)
         regular = 0
         if self.save_every is not None:
             regular = self.regular_event_count(duration, self.save_every)
@@ -120,6 +149,8 @@ class LoopTiming:
 
     def save_stop_time(self, duration: float, settling_time: float, t0: float) -> float:
         start = np.float64(settling_time) + np.float64(t0)
+        if self.save_table:
+            return self.save_table_time(len(self.save_times), settling_time, t0)
         if self.save_every is None:
             return float(start + np.float64(self.precision(duration)))
         events = self.regular_event_count(duration, self.save_every)

@@ -164,6 +164,10 @@ struct OracleConfig {
     // duration, n_runs entries each; null when not used
     const double* rtol_per_run;
     const double* duration_per_run;
+    // explicit save times (offsets from t0 + settling) of rows 1..len, in
+    // place of save_every; null / 0 when not used
+    const double* save_table;
+    int32_t save_table_len;
 };
 
 }  // extern "C"
@@ -779,6 +783,13 @@ static int32_t run_loop(const OracleConfig* cfg, Problem<T>& P, Stepper& stepper
     int32_t save_idx = 0, summary_idx = 0, update_idx = 0;
     T next_save = T(settling + t0);
     T next_update = T(settling + t0);
+    const T save_start = next_save;
+    // time of save row k >= 1 from the explicit table; +inf past its end
+    auto table_save = [&](int32_t k) -> T {
+        return (k - 1 < cfg->save_table_len)
+                   ? T(double(save_start) + cfg->save_table[k - 1])
+                   : std::numeric_limits<T>::infinity();
+    };
 
     for (int i = 0; i < n; ++i) state[i] = init[i];
     for (int i = 0; i < cfg->n_params; ++i) params[i] = par[i];
@@ -849,7 +860,8 @@ static int32_t run_loop(const OracleConfig* cfg, Problem<T>& P, Stepper& stepper
     };
 
     if (settling == T(0)) {
-        if (cfg->save_regularly) next_save = T(next_save + save_every);
+        if (cfg->save_table_len > 0) next_save = table_save(1);
+        else if (cfg->save_regularly) next_save = T(next_save + save_every);
         if (cfg->summarise_regularly) next_update = T(sample_every + next_update);
         save_state_fn(save_idx);
         save_idx += 1;
@@ -952,7 +964,8 @@ static int32_t run_loop(const OracleConfig* cfg, Problem<T>& P, Stepper& stepper
         do_save = do_save && accept;
         do_update = do_update && accept;
         if (do_save) {
-            if (cfg->save_regularly) next_save += save_every;
+            if (cfg->save_table_len > 0) next_save = table_save(save_idx + 1);
+            else if (cfg->save_regularly) next_save += save_every;
             save_state_fn(save_idx);
             save_idx += 1;
             if (cfg->save_counters)

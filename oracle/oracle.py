@@ -100,6 +100,8 @@ class OracleConfig(ctypes.Structure):
         ("mixed", ctypes.c_int32),
         ("rtol_per_run", _D),
         ("duration_per_run", _D),
+        ("save_table", _D),
+        ("save_table_len", ctypes.c_int32),
     ]
 
 
@@ -262,6 +264,9 @@ class OracleRun:
         cfg.sample_summaries_every = timing.sample_summaries_every or 0.0
         cfg.save_last = int(timing.save_last)
         cfg.save_regularly = int(timing.save_regularly)
+        if timing.save_table:
+            cfg.save_table = _dp(arr(timing.save_times))
+            cfg.save_table_len = len(timing.save_times)
         cfg.summarise_regularly = int(timing.summarise_regularly)
         cfg.samples_per_summary = timing.samples_per_summary if oc.save_summaries else 0
         cfg.save_state = int(oc.save_state)

@@ -85,3 +85,53 @@ def test_waves_per_eu_in_source():
     assert src.count("amdgpu_waves_per_eu(2)") == 2      # both kernels
     s0 = ca.Solver(van_der_pol(np.float64), **KW)
     assert "amdgpu_waves_per_eu" not in s0.kernel.spec_for(5.0).source()
+
+
+# -- explicit save-time tables ----------------------------------------------
+
+def _robertson_table_solver(**kw):
+    from systems import robertson
+    base = dict(algorithm="ros3p", atol=1e-10, rtol=1e-6, linear_correction_type="bicgstab",
+                output_types=["state", "time"])
+    base.update(kw)
+    return ca.Solver(robertson(np.float64), **base)
+
+
+def test_save_table_rows_land_on_the_table(oracle_module):
+    from systems import robertson_batch
+    inits, params = robertson_batch(6, seed=3)
+    table = np.logspace(-5, 5, 200)
+    s = _robertson_table_solver(save_times=table)
+    r = oracle_module.OracleRun(s.kernel, "robertson", 1e5).solve(inits, params, threads=4)
+    assert r["state"].shape == (201, 4, 6)
+    assert np.all(r["status_codes"] == 0)
+    assert np.array_equal(r["state"][1:, 3, :], np.repeat(table[:, None], 6, axis=1))
+    assert r["state"][0, 3, 0] == 0.0
+    # mass conservation holds on every row
+    assert np.abs(r["state"][:, :3, :].sum(axis=1) - 1.0).max() < 1e-8
+
+
+def test_uniform_table_matches_save_every(oracle_module):
+    """A table on the uniform grid k * 0.25 saves at the same instants as
+    save_every = 0.25 (binary fractions: the accumulated and the tabulated
+    event times are the same numbers), so the rows agree exactly."""
+    from systems import robertson_batch
+    inits, params = robertson_batch(4, seed=4)
+    a = _robertson_table_solver(save_every=0.25)
+    b = _robertson_table_solver(save_times=np.arange(1, 41) * 0.25)
+    ra = oracle_module.OracleRun(a.kernel, "robertson", 10.0).solve(inits, params)
+    rb = oracle_module.OracleRun(b.kernel, "robertson", 10.0).solve(inits, params)
+    assert ra["state"].shape == rb["state"].shape == (41, 4, 4)
+    assert np.array_equal(ra["state"], rb["state"])
+
+
+def test_save_table_validation():
+    with pytest.raises(ValueError):
+        _robertson_table_solver(save_times=[1.0, 0.5])
+    with pytest.raises(ValueError):
+        _robertson_table_solver(save_times=[0.0, 1.0])
+    with pytest.raises(ValueError):
+        _robertson_table_solver(save_every=1.0, save_times=[1.0, 2.0])
+    s = _robertson_table_solver(save_times=[1.0, 20.0])
+    with pytest.raises(ValueError):
+        s.kernel._validate_timing(s.kernel.timing, 10.0)
