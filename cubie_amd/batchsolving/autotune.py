@@ -46,12 +46,13 @@ def autotune(solver, inits, params, duration: float, *, sample_runs: int = 13107
              waves_per_eu: Iterable[int] = (0,),
              blocks_per_cu: Iterable[int] = (1, 2, 4, 8),
              kernel_mode: str = "persistent", rtol_per_run=None, duration_per_run=None,
-             apply: bool = True) -> dict:
+             apply: bool = True, report=None) -> dict:
     """Time every candidate launch shape on a sample of the batch and return
     ``{"block_size", "waves_per_eu", "blocks_per_cu", "ms", "table", ...}``.
 
     With ``apply`` the winner is written into the solver's kernel settings,
-    so later ``solve`` calls launch with it.
+    so later ``solve`` calls launch with it. ``report``, if given, is called
+    with each candidate's row as soon as it is timed.
     """
     kernel = getattr(solver, "kernel", solver)
     ins_all = np.asarray(inits) if not hasattr(inits, "cpu") else inits
@@ -83,6 +84,8 @@ def autotune(solver, inits, params, duration: float, *, sample_runs: int = 13107
                 vgprs = info["module"].info().get("vgprs") if "module" in info else None
                 table.append({"block_size": bs, "waves_per_eu": w, "blocks_per_cu": bpc,
                               "ms": float(ms), "vgprs": vgprs})
+                if report is not None:
+                    report(table[-1])
         finally:
             kernel.block_size, kernel.waves_per_eu, kernel.blocks_per_cu = old
         win = min(table, key=lambda r: r["ms"])

@@ -60,8 +60,12 @@ struct FirkProblem {
     Vec<L, (cache ? s * n * n : 1)> J;   // J(Y_i) per stage, refreshed per linear solve
     Vec<L, n> kw;                        // 1 / Krylov tolerance weights (per state)
 
-    // Called by the Newton iteration after each residual (fixed iterate
+    // Residual, then the per-stage Jacobians at the same iterate (fixed
     // during the Krylov solve that follows).
+    __device__ __forceinline__ void residual_prepare(real* out) {
+        residual(out);
+        prepare_linear();
+    }
     __device__ __forceinline__ void prepare_linear() {
         if constexpr (cache) {
 #pragma unroll

@@ -296,8 +296,9 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
     for (int k = 0; k < Cfg::newton_max_iters; ++k) {
         if (converged || failed) break;
         ++iteration;
-        P.residual(residual);
-        P.prepare_linear();
+        // residual at the iterate and, for the Krylov solve that follows
+        // (a fixed iterate, hence a fixed Jacobian), the problem's linear data
+        P.residual_prepare(residual);
         int kit = 0;
         int lin;
         if constexpr (same_type<L, real>::value) {
@@ -406,18 +407,33 @@ struct DirkStageProblem {
 #pragma unroll
         for (int i = 0; i < n; ++i) Y[i] = L(base[i] + a * u[i]);
     }
-    // Called by the Newton iteration after each residual: the Krylov solve
-    // that follows sees a fixed iterate, hence a fixed Jacobian.
-    __device__ __forceinline__ void prepare_linear() {
-        if constexpr (cache) {
-            L Y[n];
-            stage_state_l(Y);
-            Sys::jac(Y, params_l, drivers_l, J.data(), L(t));
-        }
+    // Reciprocal Krylov weights; the base is fixed for the whole stage solve.
+    __device__ __forceinline__ void init_weights() {
 #pragma unroll
         for (int i = 0; i < n; ++i) {
             L tol = L(Cfg::krylov_atol[i]) + L(Cfg::krylov_rtol[i]) * dabs(L(base[i]));
             kw[i] = L(1) / pmax(tol, L(1e-16));
+        }
+    }
+    // Residual u - h f(base + a u) and the Jacobian at the same stage state.
+    // With one precision throughout, f and J come from one evaluation that
+    // shares their transcendental terms; otherwise J is formed in the
+    // linear-solve precision from the rounded stage state.
+    __device__ __forceinline__ void residual_prepare(real* out) {
+        if constexpr (cache && same_type<L, real>::value) {
+            real Y[n], f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1];
+#pragma unroll
+            for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
+            Sys::dxdt_jac(Y, params, drivers, obs, f, J.data(), t);
+#pragma unroll
+            for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
+        } else {
+            residual(out);
+            if constexpr (cache) {
+                L Y[n];
+                stage_state_l(Y);
+                Sys::jac(Y, params_l, drivers_l, J.data(), L(t));
+            }
         }
     }
     __device__ __forceinline__ void jv(const L* v, L* out) const {

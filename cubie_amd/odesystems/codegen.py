@@ -14,6 +14,10 @@ emitted as HIP C++ for hiprtc:
 * ``jac(y, p, d, J, t)``           dense J (row-major) and its structural
                                    nonzero mask ``jac_nz``
 * ``jacp(y, p, d, Jp, t)``         dense df/dparams (row-major)
+* ``dxdt_jac(y, p, d, obs, dy, J, t)`` f and the dense J from one evaluation
+                                   (shared subexpressions: the exponentials of
+                                   f are those J needs), for Newton iterations
+                                   that evaluate both at the same iterate
 
 and, for the discrete adjoint (``adjoint=True``):
 
@@ -386,6 +390,18 @@ class SystemCodegen:
         parts += self._bindings()
         parts += self._emit_block(J, "jac")
         parts.append("    (void)y; (void)p; (void)d; (void)J; (void)t;")
+        parts.append("  }")
+        # f and J in one evaluation
+        fj = [(f"dy[{i}]", e) for i, e in enumerate(dx)]
+        fj += [(f"obs[{i}]", e) for i, e in enumerate(obs)]
+        fj += [(f"J[{i * n + j}]", sp.diff(dx[i], s.state_symbols[j]))
+               for i in range(n) for j in range(n)]
+        parts.append("  template <typename real> __device__ static __forceinline__ void dxdt_jac(const real* __restrict__ y, "
+                     "const real* __restrict__ p, const real* __restrict__ d, real* __restrict__ obs, "
+                     "real* __restrict__ dy, real* __restrict__ J, real t) {")
+        parts += self._bindings()
+        parts += self._emit_block(fj, "dxdt_jac")
+        parts.append("    (void)y; (void)p; (void)d; (void)obs; (void)dy; (void)J; (void)t;")
         parts.append("  }")
         npar = s.num_parameters
         Jp = [(f"Jp[{i * max(npar, 1) + j}]", sp.diff(dx[i], s.parameter_symbols[j]))

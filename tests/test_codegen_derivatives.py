@@ -26,6 +26,8 @@ void c_dxdt(const double* y, const double* p, const double* d, double* dy, doubl
   double obs[64]; System::dxdt<double>(y, p, d, obs, dy, t); }
 void c_jac(const double* y, const double* p, const double* d, double* J, double t) {
   System::jac<double>(y, p, d, J, t); }
+void c_dxdt_jac(const double* y, const double* p, const double* d, double* dy, double* J, double t) {
+  double obs[64]; System::dxdt_jac<double>(y, p, d, obs, dy, J, t); }
 void c_jacp(const double* y, const double* p, const double* d, double* J, double t) {
   System::jacp<double>(y, p, d, J, t); }
 void c_dfdt(const double* y, const double* p, const double* d, const double* dd, double* o, double t) {
@@ -97,6 +99,10 @@ def test_generated_derivatives_match_finite_differences(case, tmp_path):
     J = hs.call("c_jac", y, p, d, outs=[n * n])[0].reshape(n, n)
     Jfd = _fd(lambda yy: hs.f(yy, p, d), y).T
     assert np.abs(J - Jfd).max() <= 1e-6 * max(np.abs(Jfd).max(), 1.0)
+    # the fused evaluation returns the same f and J
+    f2, J2 = hs.call("c_dxdt_jac", y, p, d, outs=[n, n * n])
+    assert np.allclose(f2, f0, rtol=1e-14, atol=1e-14 * scale)
+    assert np.allclose(J2.reshape(n, n), J, rtol=1e-13, atol=1e-13 * max(np.abs(J).max(), 1.0))
     if npar:
         Jp = hs.call("c_jacp", y, p, d, outs=[n * npar])[0].reshape(n, npar)
         Jpfd = _fd(lambda pp: hs.f(y, pp, d), p).T
