@@ -124,3 +124,25 @@ def test_lorenz_compensated_persistent_matches_plain_kernel():
     rb = b.solve(inits, params, duration=2.0, nan_error_trajectories=False)
     assert np.array_equal(ra.state, rb.state)
     assert np.array_equal(ra.iteration_counters, rb.iteration_counters)
+
+
+def test_robertson_radau_compensated_long_horizon(oracle_module):
+    """Radau IIA on Robertson over [0, 1e5] in the mixed mode: every run
+    finishes, agrees with the oracle's mixed restatement, and stays within
+    the tolerance-level distance of FP64 (no run reports success with a
+    wrong state)."""
+    n = 128
+    inits, params = robertson_batch(n, seed=46)
+    kw = dict(algorithm="radau", atol=1e-10, rtol=1e-6, linear_correction_type="bicgstab",
+              save_times=np.logspace(-5, 5, 40), output_types=["state"])
+    solver = ca.Solver(robertson(np.float32), precision_mode="compensated", **kw)
+    res = solver.solve(inits, params, duration=1e5, nan_error_trajectories=False)
+    assert np.all(res.status_codes == 0), res.status_messages
+    mixed = oracle_module.OracleRun(solver.kernel, "robertson", 1e5).solve(inits, params, threads=8)
+    assert np.array_equal(res.status_codes, mixed["status_codes"])
+    ref = fp64_oracle(oracle_module, robertson(np.float64), kw, "robertson", inits,
+                      params.astype(np.float32).astype(np.float64), 1e5)
+    a, b = res.state[-1].astype(np.float64), ref["state"][-1]
+    e = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+    print("radau robertson compensated long horizon: max", e.max())
+    assert e.max() <= 1e-6

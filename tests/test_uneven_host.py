@@ -135,3 +135,22 @@ def test_save_table_validation():
     s = _robertson_table_solver(save_times=[1.0, 20.0])
     with pytest.raises(ValueError):
         s.kernel._validate_timing(s.kernel.timing, 10.0)
+
+
+def test_mixed_mode_radau_robertson_long_horizon(oracle_module):
+    """The oracle's mixed mode (FP64 Newton, FP32 Krylov) on Robertson with
+    Radau IIA over [0, 1e5]: FP32 solves that make no progress are not
+    applied as Newton corrections, so every run finishes and lands within
+    the tolerance-level distance of FP64."""
+    from systems import robertson, robertson_batch
+    inits, params = robertson_batch(96, seed=9)
+    kw = dict(algorithm="radau", atol=1e-10, rtol=1e-6, linear_correction_type="bicgstab",
+              save_times=np.logspace(-5, 5, 40), output_types=["state"])
+    s32 = ca.Solver(robertson(np.float32), precision_mode="compensated", **kw)
+    s64 = ca.Solver(robertson(np.float64), **kw)
+    rm = oracle_module.OracleRun(s32.kernel, "robertson", 1e5).solve(inits, params, threads=8)
+    r64 = oracle_module.OracleRun(s64.kernel, "robertson", 1e5).solve(
+        inits, params.astype(np.float32).astype(np.float64), threads=8)
+    assert np.all(rm["status_codes"] == 0)
+    e = np.linalg.norm(rm["state"][-1] - r64["state"][-1], axis=0) / np.linalg.norm(r64["state"][-1], axis=0)
+    assert e.max() <= 1e-6, e.max()
