@@ -301,7 +301,7 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         P.residual_prepare(residual);
         int kit = 0;
         int lin;
-        bool progress = false;   // mixed mode: the FP32 solve at least halved the residual
+        bool progress = false;   // mixed mode: the FP32 solve lowered the residual norm
         if constexpr (same_type<L, real>::value) {
 #pragma unroll
             for (int i = 0; i < W; ++i) {
@@ -319,7 +319,7 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
             const L n0 = P.krylov_norm2(rl);
             lin = linear_solve<Cfg, L>(P, rl, dl, true, kit);
             // the solvers leave the running residual in rl
-            progress = P.krylov_norm2(rl) <= L(0.25) * n0;
+            progress = P.krylov_norm2(rl) < n0;
 #pragma unroll
             for (int i = 0; i < W; ++i) delta[i] = real(dl[i]);
         }
@@ -331,12 +331,11 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         const bool judged = lin == ST_SUCCESS;
         // FP32 Krylov solves of the mixed mode: a solve that stops short
         // (iteration cap, or a BiCGSTAB breakdown where the operator's
-        // symmetric part is indefinite) but at least halved the residual
-        // norm still returns a useful correction. It is applied, but it
-        // neither ends the iteration nor enters the contraction estimate
-        // (the next converged correction starts a fresh history). Solves
-        // that made no such progress are not applied: on stiff kinetics
-        // (Robertson, Radau IIA) their corrections point the wrong way.
+        // symmetric part is indefinite) but lowered the residual norm still
+        // returns a useful correction. It is applied, but it neither ends
+        // the iteration nor enters the contraction estimate (the next
+        // converged correction starts a fresh history). Solves that made no
+        // progress are not applied.
         const bool partial = Cfg::newton_partial && !judged && progress &&
                              (lin == ST_LINEAR_MAX_ITERS || lin == ST_BICGSTAB_BREAKDOWN);
         const bool usable = judged || partial;

@@ -36,6 +36,9 @@ DEVICE_HEADERS = [
     "erk.hpp", "dirk.hpp", "firk.hpp", "rosenbrock.hpp", "euler_implicit.hpp", "loop.hpp",
 ]
 
+# Families whose mixed-precision mode solves the Krylov systems in FP32.
+FP32_KRYLOV_FAMILIES = ("dirk", "backwards_euler", "backwards_euler_pc", "crank_nicolson")
+
 # Step adjoints of the discrete adjoint (adjoint.hpp), by family.
 FAMILY_ADJOINT = {"erk": "ErkAdjoint", "rosenbrock": "RosenbrockAdjoint"}
 # Largest state count of the Rosenbrock adjoint's dense stage solves.
@@ -135,12 +138,13 @@ class KernelSpec:
     @property
     def fp32_krylov(self) -> bool:
         """Whether the Krylov solves run in FP32. In the mixed mode the
-        Newton-type families (DIRK, FIRK, backward Euler, Crank-Nicolson)
-        solve in FP32 and correct against an FP64 residual; a Rosenbrock
-        stage is a single linear solve with no outer correction, whose
-        accuracy FP32 cannot guarantee on stiff systems (the FP32 attainable
-        residual sits above the stopping test), so it stays in FP64."""
-        return self.mixed_implicit and self.algorithm.family != "rosenbrock"
+        single-stage Newton families (DIRK, backward Euler, Crank-Nicolson)
+        solve in FP32 and correct against an FP64 residual. A Rosenbrock
+        stage is a single linear solve with no outer correction, and the
+        coupled s*n FIRK systems are conditioned far worse than one stage's
+        n*n system: on stiff kinetics (Robertson) FP32 cannot reach their
+        stopping tests, so both stay in FP64."""
+        return self.mixed_implicit and self.algorithm.family in FP32_KRYLOV_FAMILIES
 
     @property
     def work_precision(self):

@@ -216,12 +216,13 @@ class OracleRun:
             return a
 
         # The mixed-precision (compensated) mode of an implicit family keeps the
-        # loop in FP64 and, for the Newton-type families, solves the Krylov
-        # systems in FP32 (Rosenbrock stages, which have no outer correction,
-        # stay in FP64); inputs and outputs stay in the system precision.
+        # loop in FP64 and, for the single-stage Newton families, solves the
+        # Krylov systems in FP32 (Rosenbrock stages and the coupled FIRK
+        # systems stay in FP64); inputs and outputs stay in the system precision.
         self.mixed = (getattr(k, "precision_mode", "plain") == "compensated"
                       and alg.is_implicit)
-        fp32_krylov = self.mixed and alg.family != "rosenbrock"
+        from cubie_amd.integrators.kernel_source import FP32_KRYLOV_FAMILIES
+        fp32_krylov = self.mixed and alg.family in FP32_KRYLOV_FAMILIES
         self.io_dtype = prec
         cfg.precision_bits = 64 if (self.mixed or prec is np.float64) else 32
         cfg.mixed = int(fp32_krylov)
