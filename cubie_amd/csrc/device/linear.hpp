@@ -301,7 +301,6 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         P.residual_prepare(residual);
         int kit = 0;
         int lin;
-        bool progress = false;   // mixed mode: the FP32 solve lowered the residual norm
         if constexpr (same_type<L, real>::value) {
 #pragma unroll
             for (int i = 0; i < W; ++i) {
@@ -316,10 +315,7 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
                 rl[i] = L(-residual[i]);
                 dl[i] = L(0);
             }
-            const L n0 = P.krylov_norm2(rl);
             lin = linear_solve<Cfg, L>(P, rl, dl, true, kit);
-            // the solvers leave the running residual in rl
-            progress = P.krylov_norm2(rl) < n0;
 #pragma unroll
             for (int i = 0; i < W; ++i) delta[i] = real(dl[i]);
         }
@@ -329,14 +325,14 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         const real norm2_dz = P.correction_norm2(delta);
         const real ndz = real(dsqrt(norm2_dz));
         const bool judged = lin == ST_SUCCESS;
-        // FP32 Krylov solves of the mixed mode: a solve that stops short
-        // (iteration cap, or a BiCGSTAB breakdown where the operator's
-        // symmetric part is indefinite) but lowered the residual norm still
-        // returns a useful correction. It is applied, but it neither ends
-        // the iteration nor enters the contraction estimate (the next
-        // converged correction starts a fresh history). Solves that made no
-        // progress are not applied.
-        const bool partial = Cfg::newton_partial && !judged && progress &&
+        // FP32 Krylov solves of the mixed mode (single-stage Newton
+        // families): a solve that stops short (iteration cap, or a BiCGSTAB
+        // breakdown where the operator's symmetric part is indefinite and an
+        // FP32 inner product rounds to 0) still returns a correction. It is
+        // applied, which also moves the next solve off the breakdown, but it
+        // neither ends the iteration nor enters the contraction estimate (the
+        // next converged correction starts a fresh history).
+        const bool partial = Cfg::newton_partial && !judged &&
                              (lin == ST_LINEAR_MAX_ITERS || lin == ST_BICGSTAB_BREAKDOWN);
         const bool usable = judged || partial;
         const bool history = ndz_prev > real(0);

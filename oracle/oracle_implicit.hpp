@@ -327,17 +327,12 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         for (int i = 0; i < W; ++i) { res[i] = -res[i]; delta[i] = T(0); }
         int kit = 0;
         int32_t lin;
-        bool progress = false;
         if (cfg->mixed) {
-            // FP32 Krylov solve of the FP64 Newton system; the solvers leave
-            // the running residual in rl, so progress = the weighted residual
-            // norm fell below its starting value
+            // FP32 Krylov solve of the FP64 Newton system
             float rl[MAXW], dl[MAXW];
             for (int i = 0; i < W; ++i) { rl[i] = float(res[i]); dl[i] = 0.0f; }
             LinView<Prob, float> lv{&Pb, cfg};
-            const float n0 = lv.krylov_norm2(rl);
             lin = linear_solve<float>(cfg, lv, W, rl, dl, true, kit);
-            progress = lv.krylov_norm2(rl) < n0;
             for (int i = 0; i < W; ++i) delta[i] = T(dl[i]);
         } else {
             lin = linear_solve<T>(cfg, Pb, W, res, delta, true, kit);
@@ -349,10 +344,10 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         T ndz = T(std::sqrt(n2));
         bool judged = lin == ST_SUCCESS;
         // Mixed mode only (not in the reference): an FP32 solve that stops
-        // short at the iteration cap or a BiCGSTAB breakdown, having lowered
-        // the residual norm, still yields a correction; it is applied, but
-        // neither ends the iteration nor enters the contraction estimate.
-        bool partial = cfg->mixed && !judged && progress &&
+        // short at the iteration cap or a BiCGSTAB breakdown still yields a
+        // correction; it is applied, but neither ends the iteration nor
+        // enters the contraction estimate.
+        bool partial = cfg->mixed && !judged &&
                        (lin == ST_LINEAR_MAX_ITERS || lin == ST_BICGSTAB_BREAKDOWN);
         bool usable = judged || partial;
         bool hist = ndz_prev > T(0);
