@@ -383,9 +383,13 @@ __device__ __forceinline__ void integrate_record(const cubie_adjoint_args& A) {
 }
 
 // Gradient contributions of the outputs taken at one step's end state.
+// Max / min summaries: walking a window's samples last to first, the first
+// sample whose tape flag says it set the running extremum is the one the
+// forward pass kept (`found` marks the pairs already assigned in `window`).
 template <typename Cfg>
 __device__ __forceinline__ void add_output_grads(const cubie_adjoint_args& A, long long run,
                                                  unsigned flags, long long& save_i, long long& upd_i,
+                                                 unsigned& found, long long& window,
                                                  const typename Cfg::real* y_end,
                                                  typename Cfg::real* lam) {
     using real = typename Cfg::real;
@@ -412,7 +416,21 @@ __device__ __forceinline__ void add_output_grads(const cubie_adjoint_args& A, lo
             const long long row = upd_i / Cfg::samples_per_summary;
             const io_real* g = static_cast<const io_real*>(A.grad_state_summ);
             const io_real* out = static_cast<const io_real*>(a.state_summ_out);
+            if (row != window) {
+                window = row;
+                found = 0u;
+            }
             if (g != nullptr && upd_i >= 0 && row < a.summary_rows) {
+#pragma unroll
+                for (int k = 0; k < Cfg::adj_ext_count; ++k) {
+                    const unsigned bit = 4u << k;
+                    if ((flags & bit) && !(found & bit)) {
+                        found |= bit;
+                        const int v = Cfg::adj_ext_var[k];
+                        const long long col = (row * (NS * OH) + v * OH + Cfg::adj_ext_out[k]) * N + run;
+                        lam[Cfg::summ_state_idx[v]] += real(g[col]);
+                    }
+                }
                 const real inv_spp = real(1.0 / double(Cfg::samples_per_summary));
 #pragma unroll
                 for (int v = 0; v < NS; ++v) {
@@ -428,8 +446,6 @@ __device__ __forceinline__ void add_output_grads(const cubie_adjoint_args& A, lo
                         } else if (id == M_RMS) {
                             const real r = real(out[col]);
                             lam[si] += r != real(0) ? real(g[col]) * x * inv_spp / r : real(0);
-                        } else if (id == M_MAX || id == M_MIN) {
-                            lam[si] += (io_real(x) == out[col]) ? real(g[col]) : real(0);
                         }
                         ooff += Cfg::metric_out_sizes[m];
                     }
@@ -474,6 +490,8 @@ __device__ __forceinline__ void adjoint_backward(const cubie_adjoint_args& A) {
     real* rep_k = static_cast<real*>(A.replay_stages);
     double* rep_t = A.replay_t;
     const long long nseg = (steps + K - 1) / K;
+    unsigned found = 0u;
+    long long window = -1;
 
     for (long long seg = nseg - 1; seg >= 0; --seg) {
         const long long k0 = seg * K;
@@ -501,7 +519,7 @@ __device__ __forceinline__ void adjoint_backward(const cubie_adjoint_args& A) {
                 real y_end[n];
 #pragma unroll
                 for (int c = 0; c < n; ++c) y_end[c] = rep_y[((long long)(j + 1) * n + c) * N + run];
-                add_output_grads<Cfg>(A, run, flags, save_i, upd_i, y_end, lam);
+                add_output_grads<Cfg>(A, run, flags, save_i, upd_i, found, window, y_end, lam);
             }
             real y0[n], kst[S];
 #pragma unroll
@@ -518,7 +536,7 @@ __device__ __forceinline__ void adjoint_backward(const cubie_adjoint_args& A) {
         real y0[n];
 #pragma unroll
         for (int c = 0; c < n; ++c) y0[c] = ck[c * N + run];
-        add_output_grads<Cfg>(A, run, 1u, save_i, upd_i, y0, lam);
+        add_output_grads<Cfg>(A, run, 1u, save_i, upd_i, found, window, y0, lam);
     }
     io_real* gi = static_cast<io_real*>(A.grad_inits);
     io_real* gp = static_cast<io_real*>(A.grad_params);

@@ -428,6 +428,18 @@ struct DirkStageProblem {
             Sys::dxdt_jac(Y, params, drivers, obs, f, J.data(), t);
 #pragma unroll
             for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
+        } else if constexpr (cache && Cfg::mixed_fused_jac) {
+            // mixed mode: f and J from one FP64 evaluation, J rounded to the
+            // linear-solve precision
+            real Y[n], f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1], Jr[n * n];
+#pragma unroll
+            for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
+            Sys::dxdt_jac(Y, params, drivers, obs, f, Jr, t);
+#pragma unroll
+            for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
+#pragma unroll
+            for (int i = 0; i < n * n; ++i)
+                J[i] = Sys::jac_nz[i] ? L(Jr[i]) : L(0);
         } else {
             residual(out);
             if constexpr (cache) {

@@ -36,7 +36,9 @@ struct LoopCtx {
 // Recording buffers of the discrete adjoint's forward pass (the adjoint
 // kernels are in adjoint.hpp): per system, the size of every accepted step
 // and whether a save row / summary sample was taken at its end, and the
-// state and time every Cfg::adj_ckpt_every accepted steps (and at t0).
+// state and time every Cfg::adj_ckpt_every accepted steps (and at t0);
+// flag bits 2.. mark the samples that set a max / min summary's running
+// extremum, so the backward pass finds each window's argmax exactly.
 template <typename Cfg>
 struct Tape {
     using real = typename Cfg::real;
@@ -73,12 +75,13 @@ struct Tape {
         }
     }
 
+    // extra: bits 2.. (max / min extrema set by this step's sample)
     template <typename S>
-    __device__ __forceinline__ void step(real h, bool saved, bool sampled, const S& state_after,
-                                         double t_after) {
+    __device__ __forceinline__ void step(real h, bool saved, bool sampled, unsigned extra,
+                                         const S& state_after, double t_after) {
         if (steps < max_steps) {
             dt[steps * N + run] = h;
-            flags[steps * N + run] = (unsigned char)((saved ? 1 : 0) | (sampled ? 2 : 0));
+            flags[steps * N + run] = (unsigned char)((saved ? 1u : 0u) | (sampled ? 2u : 0u) | extra);
         } else {
             full = true;
         }
@@ -294,7 +297,11 @@ struct Integrator {
         do_save = do_save && accept;
         do_update = do_update && accept;
         if constexpr (Record) {
-            if (accept) tape.step(dt_eff, do_save, do_update, state, t);
+            unsigned extra = 0u;
+            if constexpr (Cfg::summarise_state && Cfg::adj_ext_count > 0) {
+                if (do_update) extra = summ.extremum_flags(state);
+            }
+            if (accept) tape.step(dt_eff, do_save, do_update, extra, state, t);
         }
         if (do_save) {
             if constexpr (Cfg::save_table) next_save = table_save(a, save_idx + 1);

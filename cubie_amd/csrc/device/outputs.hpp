@@ -235,6 +235,21 @@ struct Summaries {
         }
     }
 
+    // Bits 2.. of the adjoint's flag byte: which max / min running extrema
+    // the sample `state` is about to set (taken before update()).
+    template <typename S>
+    __device__ __forceinline__ unsigned extremum_flags(const S& state) const {
+        unsigned bits = 0u;
+#pragma unroll
+        for (int k = 0; k < Cfg::adj_ext_count; ++k) {
+            const real x = state[Cfg::summ_state_idx[Cfg::adj_ext_var[k]]];
+            const real b = sbuf[Cfg::adj_ext_var[k] * H + Cfg::adj_ext_buf[k]];
+            const bool set = Cfg::adj_ext_max[k] ? (x > b) : (x < b);
+            bits |= set ? (4u << k) : 0u;
+        }
+        return bits;
+    }
+
     template <typename S, typename O>
     __device__ __forceinline__ void update(const S& state, const O& obs, int idx) {
         if constexpr (Cfg::summarise_state) {

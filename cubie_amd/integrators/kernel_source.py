@@ -126,6 +126,7 @@ class KernelSpec:
     block_size: int = 256
     waves_per_eu: int = 0
     adjoint: bool = False           # also emit the discrete-adjoint kernels
+    mixed_fused_jac: bool = True    # mixed mode: J from the FP64 f evaluation (fewer VALU instructions than an FP32 J)
     adj_ckpt_every: int = 32        # accepted steps per adjoint checkpoint
 
     # -- helpers ---------------------------------------------------------------
@@ -244,6 +245,14 @@ class KernelSpec:
         a(f"  static constexpr int metric_out_sizes[{max(len(mets), 1)}] = {c_int_array(osz)};")
         a(f"  static constexpr int summ_buf_height = {sum(bsz)};")
         a(f"  static constexpr int summ_out_height = {sum(osz)};")
+        # (summarised state, max/min metric) pairs whose running extremum the
+        # adjoint's tape records per sample (flag bits 2..7)
+        ext = self.adjoint_extrema()
+        a(f"  static constexpr int adj_ext_count = {len(ext)};")
+        a(f"  static constexpr int adj_ext_var[{max(len(ext), 1)}] = {c_int_array([e[0] for e in ext])};")
+        a(f"  static constexpr int adj_ext_buf[{max(len(ext), 1)}] = {c_int_array([e[1] for e in ext])};")
+        a(f"  static constexpr int adj_ext_out[{max(len(ext), 1)}] = {c_int_array([e[2] for e in ext])};")
+        a(f"  static constexpr int adj_ext_max[{max(len(ext), 1)}] = {c_int_array([e[3] for e in ext])};")
         # drivers
         d = self.drivers
         if d is not None and s.num_drivers > 0:
@@ -285,6 +294,7 @@ class KernelSpec:
             a(f"  static constexpr real gamma_stages[{st}] = {c_array(tab.gamma_stages, wp)};")
         a(f"  static constexpr bool be_predictor = {str(alg.family == 'backwards_euler_pc').lower()};")
         a(f"  static constexpr int adj_ckpt_every = {int(self.adj_ckpt_every)};")
+        a(f"  static constexpr bool mixed_fused_jac = {str(bool(self.mixed_fused_jac)).lower()};")
         # implicit solver settings
         im = self.implicit
         lin = {"minimal_residual": 0, "steepest_descent": 1, "bicgstab": 2}[im.linear_correction_type]
@@ -413,6 +423,26 @@ class KernelSpec:
                 f"{ROSENBROCK_ADJOINT_MAX_N} states")
         if self.compensated:
             raise NotImplementedError("the discrete adjoint does not support precision_mode='compensated'")
+
+    def adjoint_extrema(self):
+        """(variable, buffer offset, output offset, is_max) of every max / min
+        summary of a summarised state; the adjoint records which sample set
+        each running extremum. At most 6 fit the tape's flag byte."""
+        from ..outputhandling.summary_metrics import summary_metrics
+        out = self.outputs
+        if not (self.adjoint and out.summarise_state):
+            return []
+        ext, boff, ooff = [], 0, 0
+        for m, pp in out.metrics:
+            if m in ("max", "min"):
+                for v in range(len(out.summarised_state_indices)):
+                    ext.append((v, boff, ooff, 1 if m == "max" else 0))
+            boff += summary_metrics[m].buffer_len(pp)
+            ooff += summary_metrics[m].output_len(pp)
+        if len(ext) > 6:
+            raise NotImplementedError(
+                "the adjoint tracks at most 6 (state, max/min summary) pairs")
+        return ext
 
     @property
     def adjoint_stash(self) -> int:

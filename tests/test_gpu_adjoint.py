@@ -183,3 +183,47 @@ def test_adjoint_rejects_nondifferentiable_summary():
     g = torch.ones_like(rec.outputs.state_summaries)
     with pytest.raises(NotImplementedError):
         adj.backward(rec, None, g)
+
+
+@pytest.mark.parametrize("precision,bar", [(np.float64, 1e-5), (np.float32, 2e-2)])
+def test_adjoint_saved_rows_and_max_rms_summaries(precision, bar):
+    """A loss on several saved rows and on max / rms / mean summaries of two
+    states, DOPRI5 at fixed steps: the adjoint gradient against central
+    differences of the forward solve (FP64: 1e-5; FP32: the differences
+    themselves carry ~1e-3 of rounding noise at the widened step)."""
+    import torch
+    N = 8
+    inits, params = hh_batch(N, seed=65)
+    kw = dict(algorithm="dormand-prince-54", step_controller="fixed", dt=2.0 ** -7,
+              save_every=2.0, summarise_every=5.0, sample_summaries_every=0.25,
+              output_types=["state", "mean", "max", "rms"], summarise_variables=["V", "h"])
+    s = ca.Solver(hodgkin_huxley(precision), **kw)
+    s._configure_drivers(hh_driver())
+    rec = adj.record(s.kernel, inits.astype(precision), params.astype(precision), duration=10.0)
+    out = rec.outputs
+    assert int(out.status_codes.abs().sum()) == 0
+    state = out.state.cpu().numpy().astype(np.float64)
+    summ = out.state_summaries.cpu().numpy().astype(np.float64)
+    rng = np.random.default_rng(66)
+    c_state = rng.normal(size=state.shape[:2] + (1,))
+    c_summ = rng.normal(size=summ.shape[:2] + (1,))
+    gs = np.broadcast_to(c_state, state.shape).astype(precision)
+    gss = np.broadcast_to(c_summ, summ.shape).astype(precision)
+    gi, gp, _ = adj.backward(rec, torch.from_numpy(gs.copy()), torch.from_numpy(gss.copy()))
+    grad = np.concatenate([gi.cpu().numpy(), gp.cpu().numpy()]).astype(np.float64)
+    rel = 1e-6 if precision is np.float64 else 3e-3
+    xi, xp, steps = _fd_batch(inits, params, rel=rel)
+    res = s.solve(xi.astype(precision), xp.astype(precision), duration=10.0,
+                  nan_error_trajectories=False)
+    # FD steps as the FP32 inputs actually realise them
+    X = np.concatenate([xi, xp]).astype(precision).astype(np.float64)
+    L = ((c_state * res.state.astype(np.float64)).sum(axis=(0, 1))
+         + (c_summ * res.state_summaries.astype(np.float64)).sum(axis=(0, 1)))
+    L = L.reshape(N, -1, 2)
+    m = X.shape[0]
+    Xr = X.reshape(m, N, m, 2)
+    h2 = np.stack([Xr[k, :, k, 0] - Xr[k, :, k, 1] for k in range(m)])     # (m, N)
+    fd = ((L[:, :, 0] - L[:, :, 1]).T / h2)
+    err = np.abs(grad - fd).max(axis=0) / np.abs(fd).max(axis=0)
+    print(precision.__name__, "rows + max/rms/mean summaries: adjoint vs FD", err)
+    assert err.max() <= bar
