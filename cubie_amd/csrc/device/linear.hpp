@@ -279,7 +279,7 @@ __device__ __forceinline__ int linear_solve(Prob& P, L* rhs, L* x, bool x_is_zer
 // solved in Cfg::lin_real (converted at this boundary when they differ).
 template <typename Cfg, typename Prob>
 __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* counters) {
-    using L = typename Cfg::lin_real;
+    using L = typename Prob::L;
     constexpr int W = Prob::W;
     const real kappa = real(0.01);
     const real first_bound = real(1.0e-5);
@@ -332,7 +332,7 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         // applied, which also moves the next solve off the breakdown, but it
         // neither ends the iteration nor enters the contraction estimate (the
         // next converged correction starts a fresh history).
-        const bool partial = Cfg::newton_partial && !judged &&
+        const bool partial = Cfg::newton_partial && !same_type<L, real>::value && !judged &&
                              (lin == ST_LINEAR_MAX_ITERS || lin == ST_BICGSTAB_BREAKDOWN);
         const bool usable = judged || partial;
         const bool history = ndz_prev > real(0);
@@ -363,9 +363,9 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
 
 // Parameters and one driver vector in the linear-solve precision: the
 // caller's arrays when the precisions agree, converted copies otherwise.
-template <typename Cfg, int NDrv = Cfg::n_drivers>
+template <typename Cfg, int NDrv = Cfg::n_drivers, typename LT = typename Cfg::lin_real>
 struct LinInputs {
-    using L = typename Cfg::lin_real;
+    using L = LT;
     Vec<L, Cfg::n_params> pl;
     Vec<L, NDrv> dl;
     const L* params;
@@ -386,9 +386,11 @@ struct LinInputs {
 };
 
 // Single-stage (DIRK) implicit problem: unknown u, stage state base + a*u.
-template <typename Cfg, typename Sys>
+// LT is the Krylov precision (Cfg::lin_real, or `real` for the mixed mode's
+// FP64 rescue of a failed stage solve).
+template <typename Cfg, typename Sys, typename LT = typename Cfg::lin_real>
 struct DirkStageProblem {
-    using L = typename Cfg::lin_real;
+    using L = LT;
     static constexpr int n = Cfg::n;
     static constexpr int W = n;
     real* u;               // stage increment (Newton iterate)
@@ -428,7 +430,7 @@ struct DirkStageProblem {
             Sys::dxdt_jac(Y, params, drivers, obs, f, J.data(), t);
 #pragma unroll
             for (int i = 0; i < n; ++i) out[i] = u[i] - h * f[i];
-        } else if constexpr (cache && Cfg::mixed_fused_jac) {
+        } else if constexpr (cache && Cfg::mixed_fused_jac && !same_type<L, real>::value) {
             // mixed mode: f and J from one FP64 evaluation, J rounded to the
             // linear-solve precision
             real Y[n], f[n], obs[Cfg::n_obs > 0 ? Cfg::n_obs : 1], Jr[n * n];

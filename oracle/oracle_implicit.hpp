@@ -377,6 +377,33 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
     return conv ? ST_SUCCESS : fb;
 }
 
+// Single-stage Newton solve. In the mixed mode a stage whose FP32-Krylov
+// solve fails is solved again from the same guess with FP64 Krylov solves
+// (the engine's extension; the reference has no mixed mode).
+template <typename T, typename Prob>
+static int32_t stage_newton(const OracleConfig* cfg, Prob& sp, int W, T* u, T& theta,
+                            int32_t* counters) {
+    if (!cfg->mixed) return newton_solve<T>(cfg, sp, W, theta, counters);
+    T guess[MAXW];
+    for (int i = 0; i < W; ++i) guess[i] = u[i];
+    const T theta0 = theta;
+    int32_t st = newton_solve<T>(cfg, sp, W, theta, counters);
+    if (st != ST_SUCCESS) {
+        for (int i = 0; i < W; ++i) u[i] = guess[i];
+        theta = theta0;
+        OracleConfig c64 = *cfg;
+        c64.mixed = 0;
+        const OracleConfig* saved = sp.cfg;
+        sp.cfg = &c64;
+        int32_t c2[2] = {0, 0};
+        st = newton_solve<T>(&c64, sp, W, theta, c2);
+        sp.cfg = saved;
+        counters[0] += c2[0];
+        counters[1] += c2[1];
+    }
+    return st;
+}
+
 template <typename T>
 static void fill_inner_tols(const OracleConfig* cfg, int n, T* katol, T* krtol, T* natol, T* nrtol) {
     for (int i = 0; i < n; ++i) {
@@ -463,7 +490,7 @@ struct DIRKStepper {
             sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
         }
         sp.inv_n = T(1.0 / n);
-        return newton_solve<T>(cfg, sp, n, theta, counters);
+        return stage_newton<T>(cfg, sp, n, sp.u, theta, counters);
     }
 
     int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,
@@ -1125,7 +1152,7 @@ struct BackwardEulerStepper {
             sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
         }
         sp.inv_n = T(1.0 / n);
-        int32_t status = newton_solve<T>(cfg, sp, n, theta, counters);
+        int32_t status = stage_newton<T>(cfg, sp, n, u, theta, counters);
         for (int i = 0; i < n; ++i) {
             if (!predictor) increment[i] = u[i];
             proposed[i] = u[i] + state[i];
@@ -1166,7 +1193,7 @@ struct CrankNicolsonStepper {
             sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
         }
         sp.inv_n = T(1.0 / n);
-        return newton_solve<T>(cfg, sp, n, theta, counters);
+        return stage_newton<T>(cfg, sp, n, sp.u, theta, counters);
     }
 
     int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,

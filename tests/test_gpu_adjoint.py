@@ -185,45 +185,63 @@ def test_adjoint_rejects_nondifferentiable_summary():
         adj.backward(rec, None, g)
 
 
-@pytest.mark.parametrize("precision,bar", [(np.float64, 1e-5), (np.float32, 2e-2)])
-def test_adjoint_saved_rows_and_max_rms_summaries(precision, bar):
-    """A loss on several saved rows and on max / rms / mean summaries of two
-    states, DOPRI5 at fixed steps: the adjoint gradient against central
-    differences of the forward solve (FP64: 1e-5; FP32: the differences
-    themselves carry ~1e-3 of rounding noise at the widened step)."""
-    import torch
-    N = 8
-    inits, params = hh_batch(N, seed=65)
+def _rows_and_extrema_case(precision):
     kw = dict(algorithm="dormand-prince-54", step_controller="fixed", dt=2.0 ** -7,
               save_every=2.0, summarise_every=5.0, sample_summaries_every=0.25,
               output_types=["state", "mean", "max", "rms"], summarise_variables=["V", "h"])
     s = ca.Solver(hodgkin_huxley(precision), **kw)
     s._configure_drivers(hh_driver())
-    rec = adj.record(s.kernel, inits.astype(precision), params.astype(precision), duration=10.0)
+    return s
+
+
+def _rows_and_extrema_grad(s, inits, params, c_state, c_summ):
+    import torch
+    prec = s.kernel.precision
+    rec = adj.record(s.kernel, inits.astype(prec), params.astype(prec), duration=10.0)
     out = rec.outputs
     assert int(out.status_codes.abs().sum()) == 0
-    state = out.state.cpu().numpy().astype(np.float64)
-    summ = out.state_summaries.cpu().numpy().astype(np.float64)
+    gs = np.broadcast_to(c_state, tuple(out.state.shape)).astype(prec).copy()
+    gss = np.broadcast_to(c_summ, tuple(out.state_summaries.shape)).astype(prec).copy()
+    gi, gp, _ = adj.backward(rec, torch.from_numpy(gs), torch.from_numpy(gss))
+    return np.concatenate([gi.cpu().numpy(), gp.cpu().numpy()]).astype(np.float64), out
+
+
+def test_adjoint_saved_rows_and_max_rms_summaries():
+    """A loss on several saved rows and on the max / rms / mean summaries of
+    two states, DOPRI5 at fixed steps, FP64: the adjoint gradient against
+    central differences of the forward solve."""
+    N = 8
+    inits, params = hh_batch(N, seed=65)
+    s = _rows_and_extrema_case(np.float64)
     rng = np.random.default_rng(66)
-    c_state = rng.normal(size=state.shape[:2] + (1,))
-    c_summ = rng.normal(size=summ.shape[:2] + (1,))
-    gs = np.broadcast_to(c_state, state.shape).astype(precision)
-    gss = np.broadcast_to(c_summ, summ.shape).astype(precision)
-    gi, gp, _ = adj.backward(rec, torch.from_numpy(gs.copy()), torch.from_numpy(gss.copy()))
-    grad = np.concatenate([gi.cpu().numpy(), gp.cpu().numpy()]).astype(np.float64)
-    rel = 1e-6 if precision is np.float64 else 3e-3
-    xi, xp, steps = _fd_batch(inits, params, rel=rel)
-    res = s.solve(xi.astype(precision), xp.astype(precision), duration=10.0,
-                  nan_error_trajectories=False)
-    # FD steps as the FP32 inputs actually realise them
-    X = np.concatenate([xi, xp]).astype(precision).astype(np.float64)
-    L = ((c_state * res.state.astype(np.float64)).sum(axis=(0, 1))
-         + (c_summ * res.state_summaries.astype(np.float64)).sum(axis=(0, 1)))
+    c_state = rng.normal(size=(6, 4, 1))
+    c_summ = rng.normal(size=(2, 6, 1))
+    grad, out = _rows_and_extrema_grad(s, inits, params, c_state, c_summ)
+    assert tuple(out.state.shape)[:2] == (6, 4) and tuple(out.state_summaries.shape)[:2] == (2, 6)
+    xi, xp, steps = _fd_batch(inits, params)
+    res = s.solve(xi, xp, duration=10.0, nan_error_trajectories=False)
+    L = (c_state * res.state).sum(axis=(0, 1)) + (c_summ * res.state_summaries).sum(axis=(0, 1))
     L = L.reshape(N, -1, 2)
-    m = X.shape[0]
-    Xr = X.reshape(m, N, m, 2)
-    h2 = np.stack([Xr[k, :, k, 0] - Xr[k, :, k, 1] for k in range(m)])     # (m, N)
-    fd = ((L[:, :, 0] - L[:, :, 1]).T / h2)
+    fd = ((L[:, :, 0] - L[:, :, 1]) / (2 * steps.T)).T
     err = np.abs(grad - fd).max(axis=0) / np.abs(fd).max(axis=0)
-    print(precision.__name__, "rows + max/rms/mean summaries: adjoint vs FD", err)
-    assert err.max() <= bar
+    print("rows + max/rms/mean summaries: adjoint vs FD", err)
+    assert err.max() <= 1e-5
+
+
+def test_adjoint_fp32_matches_fp64_adjoint():
+    """The FP32 adjoint (FP32 tape, replay and step adjoints) against the FP64
+    adjoint of the same problem on the same FP32-representable inputs; max /
+    min gradients land on the sample the FP32 forward pass kept, which the
+    tape records (a replayed FP32 value may differ from it in the last bit)."""
+    N = 8
+    inits, params = hh_batch(N, seed=67)
+    inits = inits.astype(np.float32).astype(np.float64)
+    params = params.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(68)
+    c_state = rng.normal(size=(6, 4, 1))
+    c_summ = rng.normal(size=(2, 6, 1))
+    g32, _ = _rows_and_extrema_grad(_rows_and_extrema_case(np.float32), inits, params, c_state, c_summ)
+    g64, _ = _rows_and_extrema_grad(_rows_and_extrema_case(np.float64), inits, params, c_state, c_summ)
+    err = np.abs(g32 - g64).max(axis=0) / np.abs(g64).max(axis=0)
+    print("fp32 adjoint vs fp64 adjoint", err)
+    assert err.max() <= 1e-3
