@@ -235,12 +235,11 @@ struct DIRKStep {
         return status;
     }
 
-    template <typename LT, typename Ctx>
-    __device__ __forceinline__ int solve_stage_in(Ctx& ctx, const Vec<real, n>& base,
-                                                  const Vec<real, n>& state, real t, real h, real a,
-                                                  int* cnt) {
-        DirkStageProblem<Cfg, Sys, LT> P;
-        LinInputs<Cfg, Cfg::n_drivers, LT> li;
+    template <typename Ctx>
+    __device__ __forceinline__ int solve_stage(Ctx& ctx, const Vec<real, n>& base, const Vec<real, n>& state,
+                                               real t, real h, real a, int* cnt) {
+        DirkStageProblem<Cfg, Sys> P;
+        LinInputs<Cfg> li;
         li.bind(ctx.params.data(), ctx.proposed_drivers.data());
         P.u = stage_increment.data();
         P.base = base.data();
@@ -254,31 +253,6 @@ struct DIRKStep {
         P.a = a;
         P.init_weights();
         return newton_solve<Cfg>(P, theta, cnt);
-    }
-
-    // The stage's Newton solve in Cfg::lin_real. In the mixed mode a stage
-    // whose FP32-Krylov solve fails (stiff kinetics can put the FP32
-    // attainable residual above the stopping test) is solved again from the
-    // same starting guess with FP64 Krylov solves before the step fails.
-    template <typename Ctx>
-    __device__ __forceinline__ int solve_stage(Ctx& ctx, const Vec<real, n>& base, const Vec<real, n>& state,
-                                               real t, real h, real a, int* cnt) {
-        if constexpr (Cfg::krylov_rescue) {
-            Vec<real, n> guess = stage_increment;
-            const real theta0 = theta;
-            int st = solve_stage_in<typename Cfg::lin_real>(ctx, base, state, t, h, a, cnt);
-            if (st != ST_SUCCESS) {
-                stage_increment = guess;
-                theta = theta0;
-                int cnt64[2] = {0, 0};
-                st = solve_stage_in<real>(ctx, base, state, t, h, a, cnt64);
-                cnt[0] += cnt64[0];
-                cnt[1] += cnt64[1];
-            }
-            return st;
-        } else {
-            return solve_stage_in<typename Cfg::lin_real>(ctx, base, state, t, h, a, cnt);
-        }
     }
 };
 

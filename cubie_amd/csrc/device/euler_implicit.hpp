@@ -14,12 +14,11 @@
 
 namespace cubie {
 
-template <typename Cfg, typename Sys, typename LT, typename Ctx>
-__device__ __forceinline__ int single_stage_solve_in(Ctx& ctx, real* u, const real* base,
-                                                     const real* state, real t, real h, real a,
-                                                     real& theta, int* counters) {
-    DirkStageProblem<Cfg, Sys, LT> P;
-    LinInputs<Cfg, Cfg::n_drivers, LT> li;
+template <typename Cfg, typename Sys, typename Ctx>
+__device__ __forceinline__ int single_stage_solve(Ctx& ctx, real* u, const real* base, const real* state,
+                                                  real t, real h, real a, real& theta, int* counters) {
+    DirkStageProblem<Cfg, Sys> P;
+    LinInputs<Cfg> li;
     li.bind(ctx.params.data(), ctx.proposed_drivers.data());
     P.u = u;
     P.base = base;
@@ -33,34 +32,6 @@ __device__ __forceinline__ int single_stage_solve_in(Ctx& ctx, real* u, const re
     P.a = a;
     P.init_weights();
     return newton_solve<Cfg>(P, theta, counters);
-}
-
-// Newton solve of the stage; in the mixed mode a failed FP32-Krylov solve
-// is repeated from the same guess with FP64 Krylov solves.
-template <typename Cfg, typename Sys, typename Ctx>
-__device__ __forceinline__ int single_stage_solve(Ctx& ctx, real* u, const real* base, const real* state,
-                                                  real t, real h, real a, real& theta, int* counters) {
-    if constexpr (Cfg::krylov_rescue) {
-        real guess[Cfg::n];
-#pragma unroll
-        for (int i = 0; i < Cfg::n; ++i) guess[i] = u[i];
-        const real theta0 = theta;
-        int st = single_stage_solve_in<Cfg, Sys, typename Cfg::lin_real>(ctx, u, base, state, t, h, a,
-                                                                         theta, counters);
-        if (st != ST_SUCCESS) {
-#pragma unroll
-            for (int i = 0; i < Cfg::n; ++i) u[i] = guess[i];
-            theta = theta0;
-            int c64[2] = {0, 0};
-            st = single_stage_solve_in<Cfg, Sys, real>(ctx, u, base, state, t, h, a, theta, c64);
-            counters[0] += c64[0];
-            counters[1] += c64[1];
-        }
-        return st;
-    } else {
-        return single_stage_solve_in<Cfg, Sys, typename Cfg::lin_real>(ctx, u, base, state, t, h, a,
-                                                                        theta, counters);
-    }
 }
 
 template <typename Cfg, typename Sys>

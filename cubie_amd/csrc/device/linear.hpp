@@ -301,6 +301,9 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         P.residual_prepare(residual);
         int kit = 0;
         int lin;
+        // mixed mode: the FP32 right-hand side has a finite norm, and the
+        // FP32 solve lowered it (the solvers leave the running residual in rl)
+        bool representable = true, progress = false;
         if constexpr (same_type<L, real>::value) {
 #pragma unroll
             for (int i = 0; i < W; ++i) {
@@ -315,7 +318,10 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
                 rl[i] = L(-residual[i]);
                 dl[i] = L(0);
             }
+            const L n0 = P.krylov_norm2(rl);
+            representable = dfinite(n0);
             lin = linear_solve<Cfg, L>(P, rl, dl, true, kit);
+            progress = P.krylov_norm2(rl) < n0;
 #pragma unroll
             for (int i = 0; i < W; ++i) delta[i] = real(dl[i]);
         }
@@ -326,15 +332,22 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         const real ndz = real(dsqrt(norm2_dz));
         const bool judged = lin == ST_SUCCESS;
         // FP32 Krylov solves of the mixed mode (single-stage Newton
-        // families): a solve that stops short (iteration cap, or a BiCGSTAB
-        // breakdown where the operator's symmetric part is indefinite and an
-        // FP32 inner product rounds to 0) still returns a correction. It is
-        // applied, which also moves the next solve off the breakdown, but it
-        // neither ends the iteration nor enters the contraction estimate (the
-        // next converged correction starts a fresh history).
-        const bool partial = Cfg::newton_partial && !same_type<L, real>::value && !judged &&
-                             (lin == ST_LINEAR_MAX_ITERS || lin == ST_BICGSTAB_BREAKDOWN);
-        const bool usable = judged || partial;
+        // families): a solve that ends in a BiCGSTAB breakdown (the
+        // operator's symmetric part is indefinite and an FP32 inner product
+        // rounds to 0), or stops at the iteration cap having lowered the
+        // residual norm, still returns a correction. It is applied, which
+        // also moves the next solve off the breakdown, but it neither ends
+        // the iteration nor enters the contraction estimate (the next
+        // converged correction starts a fresh history). Any other
+        // unconverged FP32 solve (at the cap, its iterates can grow without
+        // bound on stiff kinetics), or a residual whose norm overflows FP32
+        // (a solve that "converges" at once on an infinite tolerance), fails
+        // this Newton solve at once, which rejects the step like any other
+        // Newton failure.
+        constexpr bool mixed = Cfg::newton_partial && !same_type<L, real>::value;
+        const bool partial = mixed && !judged && representable &&
+                             ((lin == ST_LINEAR_MAX_ITERS && progress) || lin == ST_BICGSTAB_BREAKDOWN);
+        const bool usable = (judged && representable) || partial;
         const bool history = ndz_prev > real(0);
         const real ndz_prev_safe = history ? ndz_prev : real(1);
         const real theta = history ? pmax(theta_decay * prev_theta, ndz / ndz_prev_safe) : prev_theta;
@@ -342,7 +355,8 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         const bool eta_accept = (theta < real(1)) && (theta * ndz < kappa * (real(1) - theta));
         const bool nonfinite = !(norm2_dz <= huge);
         const bool stagnant = judged && history && (dabs(theta - real(1)) <= stag_eps);
-        const bool diverging = (judged && history && (theta > theta_div)) || (usable && nonfinite);
+        const bool diverging = (judged && history && (theta > theta_div)) || (usable && nonfinite) ||
+                               (mixed && !usable);
         const bool converged_stagnant = stagnant && (ndz <= real(1)) && !diverging;
         const bool failed_now = diverging || (stagnant && (ndz > real(1)));
         failed = failed || failed_now;
@@ -386,8 +400,7 @@ struct LinInputs {
 };
 
 // Single-stage (DIRK) implicit problem: unknown u, stage state base + a*u.
-// LT is the Krylov precision (Cfg::lin_real, or `real` for the mixed mode's
-// FP64 rescue of a failed stage solve).
+// LT is the Krylov precision (Cfg::lin_real by default).
 template <typename Cfg, typename Sys, typename LT = typename Cfg::lin_real>
 struct DirkStageProblem {
     using L = LT;

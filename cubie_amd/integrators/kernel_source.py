@@ -127,7 +127,6 @@ class KernelSpec:
     waves_per_eu: int = 0
     adjoint: bool = False           # also emit the discrete-adjoint kernels
     mixed_fused_jac: bool = True    # mixed mode: J from the FP64 f evaluation (fewer VALU instructions than an FP32 J)
-    krylov_rescue: bool = True      # mixed mode: redo a failed stage solve with FP64 Krylov
     adj_ckpt_every: int = 32        # accepted steps per adjoint checkpoint
 
     # -- helpers ---------------------------------------------------------------
@@ -302,7 +301,6 @@ class KernelSpec:
         a(f"  static constexpr int newton_max_iters = {int(im.newton_max_iters)};")
         a(f"  static constexpr int krylov_max_iters = {int(im.krylov_max_iters)};")
         a(f"  static constexpr bool newton_partial = {'true' if self.fp32_krylov else 'false'};")
-        a(f"  static constexpr bool krylov_rescue = {'true' if (self.fp32_krylov and self.krylov_rescue) else 'false'};")
         a(f"  static constexpr int linear_solver = {lin};")
         a(f"  static constexpr int preconditioner_order = {int(im.preconditioner_order)};")
         # Small systems evaluate the dense Jacobian once per linear solve and

@@ -225,7 +225,10 @@ class OracleRun:
         fp32_krylov = self.mixed and alg.family in FP32_KRYLOV_FAMILIES
         self.io_dtype = prec
         cfg.precision_bits = 64 if (self.mixed or prec is np.float64) else 32
-        cfg.mixed = int(fp32_krylov)
+        # mixed = 2: the Krylov Jacobian is the FP64 one rounded to FP32 (the
+        # engine does this when the dense Jacobian is held, n^2 <= 64)
+        fused_jac = getattr(k, "mixed_fused_jac", True) and sysm.num_states ** 2 <= 64
+        cfg.mixed = (2 if fused_jac else 1) if fp32_krylov else 0
         cfg.system_id = SYSTEM_IDS[system_id]
         cfg.n_states = sysm.num_states
         cfg.n_params = sysm.num_parameters

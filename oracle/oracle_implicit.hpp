@@ -124,37 +124,72 @@ struct StageProblem {
     void stage_state_in(V* Y) const {
         for (int i = 0; i < n; ++i) Y[i] = V(base[i] + a * u[i]);
     }
+    // The mixed mode with mixed == 2 forms the Krylov Jacobian in T at the
+    // T stage state and rounds its entries to V (the engine's fused FP64
+    // f + J evaluation); otherwise J v is evaluated in V at the V state.
+    bool fused_jac() const { return cfg->mixed == 2; }
     template <typename V>
-    void op_in(const V* v, V* out) const {
-        V Y[MAXN], jv[MAXN];
+    void rounded_jac(V* J) const {
+        T Y[MAXN];
+        for (int i = 0; i < n; ++i) Y[i] = base[i] + a * u[i];
+        for (int j = 0; j < n; ++j) {
+            T e[MAXN], col[MAXN];
+            for (int k = 0; k < n; ++k) e[k] = T(k == j ? 1 : 0);
+            sys_jvp<T>(P->sys, Y, params, drivers, e, col, t);
+            for (int i = 0; i < n; ++i) J[i * n + j] = V(col[i]);
+        }
+    }
+    template <typename V>
+    void jv_in(const V* v, V* jv) const {
+        if (fused_jac()) {
+            ++P->work[1];
+            V J[MAXN * MAXN];
+            rounded_jac<V>(J);
+            for (int i = 0; i < n; ++i) {
+                V acc = V(0);
+                for (int j = 0; j < n; ++j)
+                    if (J[i * n + j] != V(0)) acc += J[i * n + j] * v[j];
+                jv[i] = acc;
+            }
+            return;
+        }
+        V Y[MAXN];
         InputsIn<T, V> in;
         in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
         stage_state_in<V>(Y);
         jvp_in<T, V>(P, Y, in.p, in.d, v, jv, V(t));
+    }
+    template <typename V>
+    void op_in(const V* v, V* out) const {
+        V jv[MAXN];
+        jv_in<V>(v, jv);
         const V ah = V(a * h);
         for (int i = 0; i < n; ++i) out[i] = v[i] - ah * jv[i];
     }
     template <typename V>
     void jacobi_in(const V* v, V* out) const {
-        V Y[MAXN], dg[MAXN];
-        InputsIn<T, V> in;
-        in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
-        stage_state_in<V>(Y);
-        jacobian_diagonal_in<T, V>(P, Y, in.p, in.d, V(t), dg);
+        V dg[MAXN];
+        if (fused_jac()) {
+            V J[MAXN * MAXN];
+            rounded_jac<V>(J);
+            for (int i = 0; i < n; ++i) dg[i] = J[i * n + i];
+        } else {
+            V Y[MAXN];
+            InputsIn<T, V> in;
+            in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
+            stage_state_in<V>(Y);
+            jacobian_diagonal_in<T, V>(P, Y, in.p, in.d, V(t), dg);
+        }
         const V scale = V(h * a);
         for (int i = 0; i < n; ++i) out[i] = jacobi_divide<V>(v[i], scale, dg[i]);
     }
     template <typename V>
     void neumann_in(const V* v, V* out) const {
         const V h_eff = V(h * a);
-        V Y[MAXN];
-        InputsIn<T, V> in;
-        in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
-        stage_state_in<V>(Y);
         for (int i = 0; i < n; ++i) out[i] = v[i];
         for (int k = 0; k < cfg->preconditioner_order; ++k) {
             V jv[MAXN];
-            jvp_in<T, V>(P, Y, in.p, in.d, out, jv, V(t));
+            jv_in<V>(out, jv);
             for (int i = 0; i < n; ++i) out[i] = v[i] + h_eff * jv[i];
         }
     }
@@ -327,12 +362,17 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         for (int i = 0; i < W; ++i) { res[i] = -res[i]; delta[i] = T(0); }
         int kit = 0;
         int32_t lin;
+        bool representable = true, progress = false;
         if (cfg->mixed) {
-            // FP32 Krylov solve of the FP64 Newton system
+            // FP32 Krylov solve of the FP64 Newton system; the solvers leave
+            // the running residual in rl
             float rl[MAXW], dl[MAXW];
             for (int i = 0; i < W; ++i) { rl[i] = float(res[i]); dl[i] = 0.0f; }
             LinView<Prob, float> lv{&Pb, cfg};
+            const float n0 = lv.krylov_norm2(rl);
+            representable = std::isfinite(n0);
             lin = linear_solve<float>(cfg, lv, W, rl, dl, true, kit);
+            progress = lv.krylov_norm2(rl) < n0;
             for (int i = 0; i < W; ++i) delta[i] = T(dl[i]);
         } else {
             lin = linear_solve<T>(cfg, Pb, W, res, delta, true, kit);
@@ -343,13 +383,15 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         T n2 = Pb.correction_norm2(delta);
         T ndz = T(std::sqrt(n2));
         bool judged = lin == ST_SUCCESS;
-        // Mixed mode only (not in the reference): an FP32 solve that stops
-        // short at the iteration cap or a BiCGSTAB breakdown still yields a
-        // correction; it is applied, but neither ends the iteration nor
-        // enters the contraction estimate.
-        bool partial = cfg->mixed && !judged &&
-                       (lin == ST_LINEAR_MAX_ITERS || lin == ST_BICGSTAB_BREAKDOWN);
-        bool usable = judged || partial;
+        // Mixed mode only (not in the reference): an FP32 solve that ends in
+        // a BiCGSTAB breakdown, or stops at the iteration cap having lowered
+        // the residual norm, still yields a correction; it is applied, but
+        // neither ends the iteration nor enters the contraction estimate.
+        // Any other unconverged FP32 solve, or a right-hand side whose FP32
+        // norm overflows, fails the Newton solve.
+        bool partial = cfg->mixed && !judged && representable &&
+                       ((lin == ST_LINEAR_MAX_ITERS && progress) || lin == ST_BICGSTAB_BREAKDOWN);
+        bool usable = (judged && representable) || partial;
         bool hist = ndz_prev > T(0);
         T nps = hist ? ndz_prev : T(1);
         T theta = hist ? std::max(decay * prev_theta, ndz / nps) : prev_theta;
@@ -357,7 +399,8 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         bool eta_ok = theta < T(1) && theta * ndz < kappa * (T(1) - theta);
         bool nonfinite = !(n2 <= huge);
         bool stagnant = judged && hist && std::fabs(theta - T(1)) <= stag;
-        bool diverging = (judged && hist && theta > div) || (usable && nonfinite);
+        bool diverging = (judged && hist && theta > div) || (usable && nonfinite) ||
+                         (cfg->mixed && !usable);
         bool conv_stag = stagnant && ndz <= T(1) && !diverging;
         bool failed_now = diverging || (stagnant && ndz > T(1));
         failed = failed || failed_now;
@@ -375,33 +418,6 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
     counters[0] = iters;
     counters[1] = ktot;
     return conv ? ST_SUCCESS : fb;
-}
-
-// Single-stage Newton solve. In the mixed mode a stage whose FP32-Krylov
-// solve fails is solved again from the same guess with FP64 Krylov solves
-// (the engine's extension; the reference has no mixed mode).
-template <typename T, typename Prob>
-static int32_t stage_newton(const OracleConfig* cfg, Prob& sp, int W, T* u, T& theta,
-                            int32_t* counters) {
-    if (!cfg->mixed) return newton_solve<T>(cfg, sp, W, theta, counters);
-    T guess[MAXW];
-    for (int i = 0; i < W; ++i) guess[i] = u[i];
-    const T theta0 = theta;
-    int32_t st = newton_solve<T>(cfg, sp, W, theta, counters);
-    if (st != ST_SUCCESS) {
-        for (int i = 0; i < W; ++i) u[i] = guess[i];
-        theta = theta0;
-        OracleConfig c64 = *cfg;
-        c64.mixed = 0;
-        const OracleConfig* saved = sp.cfg;
-        sp.cfg = &c64;
-        int32_t c2[2] = {0, 0};
-        st = newton_solve<T>(&c64, sp, W, theta, c2);
-        sp.cfg = saved;
-        counters[0] += c2[0];
-        counters[1] += c2[1];
-    }
-    return st;
 }
 
 template <typename T>
@@ -490,7 +506,7 @@ struct DIRKStepper {
             sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
         }
         sp.inv_n = T(1.0 / n);
-        return stage_newton<T>(cfg, sp, n, sp.u, theta, counters);
+        return newton_solve<T>(cfg, sp, n, theta, counters);
     }
 
     int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,
@@ -1152,7 +1168,7 @@ struct BackwardEulerStepper {
             sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
         }
         sp.inv_n = T(1.0 / n);
-        int32_t status = stage_newton<T>(cfg, sp, n, u, theta, counters);
+        int32_t status = newton_solve<T>(cfg, sp, n, theta, counters);
         for (int i = 0; i < n; ++i) {
             if (!predictor) increment[i] = u[i];
             proposed[i] = u[i] + state[i];
@@ -1193,7 +1209,7 @@ struct CrankNicolsonStepper {
             sp.katol[i] = katol[i]; sp.krtol[i] = krtol[i]; sp.natol[i] = natol[i]; sp.nrtol[i] = nrtol[i];
         }
         sp.inv_n = T(1.0 / n);
-        return stage_newton<T>(cfg, sp, n, sp.u, theta, counters);
+        return newton_solve<T>(cfg, sp, n, theta, counters);
     }
 
     int32_t step(const T* state, T* proposed, const T* params, T* drivers, T* prop_drivers,

@@ -154,3 +154,28 @@ def test_mixed_mode_radau_robertson_long_horizon(oracle_module):
     assert np.all(rm["status_codes"] == 0)
     e = np.linalg.norm(rm["state"][-1] - r64["state"][-1], axis=0) / np.linalg.norm(r64["state"][-1], axis=0)
     assert e.max() <= 1e-6, e.max()
+
+
+def test_mixed_mode_kvaerno5_robertson_hard_runs(oracle_module):
+    """Four runs of a 512-system Robertson batch (seed 45) on which the mixed
+    mode's FP32 BiCGSTAB returns, at its iteration cap, corrections of 1e9 to
+    1e28 tolerance units. Such a solve is not applied unless it lowered its
+    own residual, and a right-hand side whose FP32 norm overflows (a solve
+    that would "converge" at once on an infinite tolerance) fails the Newton
+    solve; the step is rejected and retried smaller, and every run finishes
+    within the tolerance-level distance of FP64 (applying those corrections
+    lands the runs on spurious stage solutions and ends them at dt_min)."""
+    from systems import robertson, robertson_batch
+    inits, params = robertson_batch(512, seed=45)
+    pick = [264, 382, 402, 464]
+    inits, params = inits[:, pick], params[:, pick]
+    kw = dict(algorithm="kvaerno5", atol=1e-10, rtol=1e-6, linear_correction_type="bicgstab",
+              save_every=0.5, output_types=["state", "iteration_counters"])
+    s32 = ca.Solver(robertson(np.float32), precision_mode="compensated", **kw)
+    s64 = ca.Solver(robertson(np.float64), **kw)
+    rm = oracle_module.OracleRun(s32.kernel, "robertson", 10.0).solve(inits, params, threads=4)
+    r64 = oracle_module.OracleRun(s64.kernel, "robertson", 10.0).solve(
+        inits, params.astype(np.float32).astype(np.float64), threads=4)
+    assert np.all(rm["status_codes"] == 0), rm["status_codes"]
+    e = np.linalg.norm(rm["state"][-1] - r64["state"][-1], axis=0) / np.linalg.norm(r64["state"][-1], axis=0)
+    assert e.max() <= 1e-5, e.max()
