@@ -37,101 +37,7 @@
 
 namespace cubie {
 
-// ---------------------------------------------------------------------------
-// Small dense LU with partial pivoting (row swaps applied as predicated
-// selects, so every index stays a compile-time constant and the factors live
-// in registers).
-// ---------------------------------------------------------------------------
-template <typename T, int n>
-struct DenseLU {
-    T a[n * n];
-    int piv[n];
-
-    __device__ __forceinline__ void factor(const T* M) {
-#pragma unroll
-        for (int i = 0; i < n * n; ++i) a[i] = M[i];
-#pragma unroll
-        for (int k = 0; k < n; ++k) {
-            int p = k;
-            T best = dabs(a[k * n + k]);
-#pragma unroll
-            for (int i = k + 1; i < n; ++i) {
-                const T v = dabs(a[i * n + k]);
-                const bool better = v > best;
-                best = better ? v : best;
-                p = better ? i : p;
-            }
-            piv[k] = p;
-#pragma unroll
-            for (int i = k + 1; i < n; ++i) {
-                const bool sw = (i == p);
-#pragma unroll
-                for (int j = 0; j < n; ++j) {
-                    const T x = a[k * n + j], y = a[i * n + j];
-                    a[k * n + j] = sw ? y : x;
-                    a[i * n + j] = sw ? x : y;
-                }
-            }
-            const T d = a[k * n + k];
-            const T inv = d != T(0) ? T(1) / d : T(0);
-#pragma unroll
-            for (int i = k + 1; i < n; ++i) {
-                const T l = a[i * n + k] * inv;
-                a[i * n + k] = l;
-#pragma unroll
-                for (int j = k + 1; j < n; ++j) a[i * n + j] -= l * a[k * n + j];
-            }
-        }
-    }
-
-    // M x = b, in place
-    __device__ __forceinline__ void solve(T* x) const {
-#pragma unroll
-        for (int k = 0; k < n; ++k) {
-#pragma unroll
-            for (int i = k + 1; i < n; ++i) {
-                const bool sw = (i == piv[k]);
-                const T u = x[k], v = x[i];
-                x[k] = sw ? v : u;
-                x[i] = sw ? u : v;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < n; ++i)
-#pragma unroll
-            for (int j = 0; j < i; ++j) x[i] -= a[i * n + j] * x[j];
-#pragma unroll
-        for (int i = n - 1; i >= 0; --i) {
-#pragma unroll
-            for (int j = i + 1; j < n; ++j) x[i] -= a[i * n + j] * x[j];
-            x[i] = x[i] / a[i * n + i];
-        }
-    }
-
-    // M^T x = b, in place (M = P^T L U: U^T, then L^T, then the swaps reversed)
-    __device__ __forceinline__ void solve_t(T* x) const {
-#pragma unroll
-        for (int i = 0; i < n; ++i) {
-#pragma unroll
-            for (int j = 0; j < i; ++j) x[i] -= a[j * n + i] * x[j];
-            x[i] = x[i] / a[i * n + i];
-        }
-#pragma unroll
-        for (int i = n - 1; i >= 0; --i)
-#pragma unroll
-            for (int j = i + 1; j < n; ++j) x[i] -= a[j * n + i] * x[j];
-#pragma unroll
-        for (int k = n - 1; k >= 0; --k) {
-#pragma unroll
-            for (int i = k + 1; i < n; ++i) {
-                const bool sw = (i == piv[k]);
-                const T u = x[k], v = x[i];
-                x[k] = sw ? v : u;
-                x[i] = sw ? u : v;
-            }
-        }
-    }
-};
+// DenseLU (partial pivoting, in registers) is defined with the linear solvers.
 
 // ---------------------------------------------------------------------------
 // Explicit Runge-Kutta step adjoint

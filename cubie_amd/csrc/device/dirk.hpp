@@ -251,6 +251,7 @@ struct DIRKStep {
         P.t = t;
         P.h = h;
         P.a = a;
+        P.use_direct = ctx.use_direct;
         P.init_weights();
         return newton_solve<Cfg>(P, theta, cnt);
     }

@@ -30,6 +30,7 @@ __device__ __forceinline__ int single_stage_solve(Ctx& ctx, real* u, const real*
     P.t = t;
     P.h = h;
     P.a = a;
+    P.use_direct = ctx.use_direct;
     P.init_weights();
     return newton_solve<Cfg>(P, theta, counters);
 }

@@ -57,6 +57,7 @@ struct FirkProblem {
     }
 
     static constexpr bool cache = Cfg::cache_jac;
+    static constexpr bool direct_capable = false;   // FP64 Krylov in the mixed mode
     Vec<L, (cache ? s * n * n : 1)> J;   // J(Y_i) per stage, refreshed per linear solve
     Vec<L, n> kw;                        // 1 / Krylov tolerance weights (per state)
 

@@ -273,20 +273,134 @@ __device__ __forceinline__ int linear_solve(Prob& P, L* rhs, L* x, bool x_is_zer
     else return mr_solve<Cfg, L>(P, rhs, x, x_is_zero, iters);
 }
 
+// ---------------------------------------------------------------------------
+// Small dense LU with partial pivoting (row swaps applied as predicated
+// selects, so every index stays a compile-time constant and the factors live
+// in registers).
+// ---------------------------------------------------------------------------
+template <typename T, int n>
+struct DenseLU {
+    T a[n * n];
+    int piv[n];
+
+    __device__ __forceinline__ void factor(const T* M) {
+#pragma unroll
+        for (int i = 0; i < n * n; ++i) a[i] = M[i];
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            int p = k;
+            T best = dabs(a[k * n + k]);
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const T v = dabs(a[i * n + k]);
+                const bool better = v > best;
+                best = better ? v : best;
+                p = better ? i : p;
+            }
+            piv[k] = p;
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const bool sw = (i == p);
+#pragma unroll
+                for (int j = 0; j < n; ++j) {
+                    const T x = a[k * n + j], y = a[i * n + j];
+                    a[k * n + j] = sw ? y : x;
+                    a[i * n + j] = sw ? x : y;
+                }
+            }
+            const T d = a[k * n + k];
+            const T inv = d != T(0) ? T(1) / d : T(0);
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const T l = a[i * n + k] * inv;
+                a[i * n + k] = l;
+#pragma unroll
+                for (int j = k + 1; j < n; ++j) a[i * n + j] -= l * a[k * n + j];
+            }
+        }
+    }
+
+    // M x = b, in place
+    __device__ __forceinline__ void solve(T* x) const {
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const bool sw = (i == piv[k]);
+                const T u = x[k], v = x[i];
+                x[k] = sw ? v : u;
+                x[i] = sw ? u : v;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+#pragma unroll
+            for (int j = 0; j < i; ++j) x[i] -= a[i * n + j] * x[j];
+#pragma unroll
+        for (int i = n - 1; i >= 0; --i) {
+#pragma unroll
+            for (int j = i + 1; j < n; ++j) x[i] -= a[i * n + j] * x[j];
+            x[i] = x[i] / a[i * n + i];
+        }
+    }
+
+    // M^T x = b, in place (M = P^T L U: U^T, then L^T, then the swaps reversed)
+    __device__ __forceinline__ void solve_t(T* x) const {
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+#pragma unroll
+            for (int j = 0; j < i; ++j) x[i] -= a[j * n + i] * x[j];
+            x[i] = x[i] / a[i * n + i];
+        }
+#pragma unroll
+        for (int i = n - 1; i >= 0; --i)
+#pragma unroll
+            for (int j = i + 1; j < n; ++j) x[i] -= a[j * n + i] * x[j];
+#pragma unroll
+        for (int k = n - 1; k >= 0; --k) {
+#pragma unroll
+            for (int i = k + 1; i < n; ++i) {
+                const bool sw = (i == piv[k]);
+                const T u = x[k], v = x[i];
+                x[k] = sw ? v : u;
+                x[i] = sw ? u : v;
+            }
+        }
+    }
+};
+
 // Newton iteration on the stage increment P.u (NLNewton acceptance rules).
 // theta_store carries the contraction estimate between solves. The residual,
 // the iterate and the correction norm are in `real`; each correction is
 // solved in Cfg::lin_real (converted at this boundary when they differ).
+//
+// Mixed mode (FP32 Krylov under an FP64 Newton iteration), for problems that
+// hold their dense Jacobian (Prob::direct_capable): when the loop repeats a
+// step after a failed one (P.use_direct), every correction is instead solved
+// directly in FP64, (I - a h J) dz = -r by LU with partial pivoting, from a
+// zero starting increment. A near-singular stage operator (a spike upstroke,
+// where 1 - a h lambda_max approaches 0) can make the FP32-rounded
+// operator's Krylov corrections diverge; the repeat is then FP64 Newton
+// with exact linear solves. A runtime switch inside the one copy of the
+// loop costs no registers in the FP32 path.
 template <typename Cfg, typename Prob>
 __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* counters) {
     using L = typename Prob::L;
     constexpr int W = Prob::W;
+    constexpr bool mixed = Cfg::newton_partial && !same_type<L, real>::value;
+    constexpr bool two_pass = mixed && Prob::direct_capable;
     const real kappa = real(0.01);
     const real first_bound = real(1.0e-5);
     const real theta_decay = real(0.3);
     const real theta_div = real(2.0);
     constexpr real huge = SolverConsts<real>::huge;
     constexpr real stag_eps = SolverConsts<real>::stagnation_eps;
+    bool direct = false;
+    if constexpr (two_pass) direct = P.use_direct;
+    if (direct) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) P.u[i] = real(0);
+    }
     real prev_theta = theta_store > real(0) ? theta_store : real(1);
     real ndz_prev = real(0);
     bool converged = false, failed = false;
@@ -296,11 +410,11 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
     for (int k = 0; k < Cfg::newton_max_iters; ++k) {
         if (converged || failed) break;
         ++iteration;
-        // residual at the iterate and, for the Krylov solve that follows
+        // residual at the iterate and, for the linear solve that follows
         // (a fixed iterate, hence a fixed Jacobian), the problem's linear data
         P.residual_prepare(residual);
         int kit = 0;
-        int lin;
+        int lin = ST_SUCCESS;
         // mixed mode: the FP32 right-hand side has a finite norm, and the
         // FP32 solve lowered it (the solvers leave the running residual in rl)
         bool representable = true, progress = false;
@@ -312,18 +426,30 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
             }
             lin = linear_solve<Cfg, L>(P, residual, delta, true, kit);
         } else {
-            L rl[W], dl[W];
+            bool done = false;
+            if constexpr (two_pass) {
+                if (direct) {
 #pragma unroll
-            for (int i = 0; i < W; ++i) {
-                rl[i] = L(-residual[i]);
-                dl[i] = L(0);
+                    for (int i = 0; i < W; ++i) delta[i] = -residual[i];
+                    lin = P.direct_solve(delta) ? ST_SUCCESS : ST_LINEAR_MAX_ITERS;
+                    kit = 1;
+                    done = true;
+                }
             }
-            const L n0 = P.krylov_norm2(rl);
-            representable = dfinite(n0);
-            lin = linear_solve<Cfg, L>(P, rl, dl, true, kit);
-            progress = P.krylov_norm2(rl) < n0;
+            if (!done) {
+                L rl[W], dl[W];
 #pragma unroll
-            for (int i = 0; i < W; ++i) delta[i] = real(dl[i]);
+                for (int i = 0; i < W; ++i) {
+                    rl[i] = L(-residual[i]);
+                    dl[i] = L(0);
+                }
+                const L n0 = P.krylov_norm2(rl);
+                representable = dfinite(n0);
+                lin = linear_solve<Cfg, L>(P, rl, dl, true, kit);
+                progress = P.krylov_norm2(rl) < n0;
+#pragma unroll
+                for (int i = 0; i < W; ++i) delta[i] = real(dl[i]);
+            }
         }
         krylov_total += kit;
         last_lin = lin;
@@ -333,19 +459,17 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
         const bool judged = lin == ST_SUCCESS;
         // FP32 Krylov solves of the mixed mode (single-stage Newton
         // families): a solve that ends in a BiCGSTAB breakdown (the
-        // operator's symmetric part is indefinite and an FP32 inner product
-        // rounds to 0), or stops at the iteration cap having lowered the
-        // residual norm, still returns a correction. It is applied, which
-        // also moves the next solve off the breakdown, but it neither ends
-        // the iteration nor enters the contraction estimate (the next
-        // converged correction starts a fresh history). Any other
-        // unconverged FP32 solve (at the cap, its iterates can grow without
-        // bound on stiff kinetics), or a residual whose norm overflows FP32
-        // (a solve that "converges" at once on an infinite tolerance), fails
-        // this Newton solve at once, which rejects the step like any other
-        // Newton failure.
-        constexpr bool mixed = Cfg::newton_partial && !same_type<L, real>::value;
-        const bool partial = mixed && !judged && representable &&
+        // operator's symmetric part is indefinite and an FP32 inner
+        // product rounds to 0), or stops at the iteration cap having
+        // lowered the residual norm, still returns a correction. It is
+        // applied, which also moves the next solve off the breakdown, but
+        // it neither ends the iteration nor enters the contraction
+        // estimate (the next converged correction starts a fresh history).
+        // Any other unconverged solve (at the cap, FP32 iterates can grow
+        // without bound on stiff kinetics), or a residual whose norm
+        // overflows FP32 (a solve that "converges" at once on an infinite
+        // tolerance), fails the pass at once.
+        const bool partial = mixed && !direct && !judged && representable &&
                              ((lin == ST_LINEAR_MAX_ITERS && progress) || lin == ST_BICGSTAB_BREAKDOWN);
         const bool usable = (judged && representable) || partial;
         const bool history = ndz_prev > real(0);
@@ -416,6 +540,11 @@ struct DirkStageProblem {
     real t, h, a;
 
     static constexpr bool cache = Cfg::cache_jac;
+    // the mixed mode's direct FP64 solves (newton_solve), for systems whose
+    // dense Jacobian the problem holds; use_direct is set for the repeat of
+    // a failed step
+    static constexpr bool direct_capable = cache && !same_type<L, real>::value;
+    bool use_direct = false;
     Vec<L, (cache ? n * n : 1)> J;   // J(base + a u), refreshed per linear solve
     Vec<L, n> kw;                    // 1 / Krylov tolerance weights (base is fixed)
 
@@ -463,6 +592,31 @@ struct DirkStageProblem {
                 Sys::jac(Y, params_l, drivers_l, J.data(), L(t));
             }
         }
+    }
+    // (I - a h J) x = b by LU with partial pivoting in FP64, b in x, from the
+    // Jacobian the FP32 pass holds (its entries are the FP64 ones rounded
+    // once; Newton tolerates that perturbation, where an FP32 Krylov solve of
+    // a near-singular operator does not); false for a zero pivot or a
+    // non-finite solution
+    __device__ __forceinline__ bool direct_solve(real* x) const {
+        real M[n * n];
+        const real ah = a * h;
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+#pragma unroll
+            for (int j = 0; j < n; ++j) {
+                const real jij = Sys::jac_nz[i * n + j] ? real(J[i * n + j]) : real(0);
+                M[i * n + j] = (i == j ? real(1) : real(0)) - ah * jij;
+            }
+        DenseLU<real, n> lu;
+        lu.factor(M);
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < n; ++k) ok = ok && lu.a[k * n + k] != real(0);
+        lu.solve(x);
+#pragma unroll
+        for (int i = 0; i < n; ++i) ok = ok && dfinite(x[i]);
+        return ok;
     }
     __device__ __forceinline__ void jv(const L* v, L* out) const {
         if constexpr (cache) {

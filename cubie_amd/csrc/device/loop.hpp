@@ -31,6 +31,9 @@ struct LoopCtx {
     Vec<real, Cfg::n_obs> observables;
     Vec<real, Cfg::n_obs> proposed_observables;
     Drivers<Cfg> drv;
+    // mixed mode: the step being attempted repeats a failed one, and its
+    // Newton corrections are solved directly in FP64
+    bool use_direct = false;
 };
 
 // Recording buffers of the discrete adjoint's forward pass (the adjoint
@@ -238,7 +241,16 @@ struct Integrator {
         const int niters = proposed_counters[0];
         iteration_status |= step_status;
         const bool step_failed = step_status != 0;
-        if constexpr (!Cfg::adaptive) irrecoverable = irrecoverable || step_failed;
+        if constexpr (Cfg::mixed_direct) {
+            // a failed step is repeated with direct FP64 Newton corrections:
+            // a fixed step once, at the same size; an adaptive one at the
+            // sizes the controller picks, until a step succeeds
+            const bool repeat = step_failed && (Cfg::adaptive || !ctx.use_direct);
+            if constexpr (!Cfg::adaptive) irrecoverable = irrecoverable || (step_failed && !repeat);
+            ctx.use_direct = repeat;
+        } else {
+            if constexpr (!Cfg::adaptive) irrecoverable = irrecoverable || step_failed;
+        }
 #pragma unroll
         for (int i = 0; i < Cfg::n_error; ++i) error[i] = step_failed ? real(1e16) : error[i];
 

@@ -309,6 +309,9 @@ class KernelSpec:
         # right-hand side for every Krylov vector.
         jac_regs = n * n * (st if alg.family == "firk" else 1)
         a(f"  static constexpr bool cache_jac = {str(jac_regs <= 64).lower()};")
+        # mixed mode: a failed step is repeated with direct FP64 Newton
+        # corrections (needs the dense Jacobian the stage problem holds)
+        a(f"  static constexpr bool mixed_direct = {str(self.fp32_krylov and jac_regs <= 64).lower()};")
         kinds = [PRECONDITIONER_KINDS[k] for k in im.preconditioner_kinds]
         a(f"  static constexpr int precond_count = {len(kinds)};")
         a(f"  static constexpr int precond_kinds[{len(kinds)}] = {c_int_array(kinds)};")

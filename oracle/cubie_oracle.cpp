@@ -235,6 +235,9 @@ struct Problem {
     DriverEval<T> drv;
     int n;
     mutable int64_t work[6] = {0, 0, 0, 0, 0, 0};
+    // mixed mode: the step being attempted repeats a failed one (direct FP64
+    // Newton corrections; the engine's extension)
+    mutable bool use_direct = false;
     void rhs(const T* y, const T* p, const T* d, T* obs, T* dy, T t) const {
         ++work[0];
         sys_rhs<T>(sys, y, p, d, obs, dy, t);
@@ -748,6 +751,9 @@ static int32_t run_loop(const OracleConfig* cfg, Problem<T>& P, Stepper& stepper
     const bool adaptive = cfg->controller != CTRL_FIXED;
     const bool summarise = cfg->summarise_state || cfg->summarise_obs;
     const int n_err = cfg->has_error ? n : 0;
+    // the mixed mode's direct repeat needs the dense Jacobian (n^2 <= 64)
+    const bool mixed_direct = cfg->mixed && n * n <= 64;
+    P.use_direct = false;
 
     ControllerState<T> ctrl;
     ctrl.init(cfg, n);
@@ -923,7 +929,16 @@ static int32_t run_loop(const OracleConfig* cfg, Problem<T>& P, Stepper& stepper
         int32_t niters = proposed_counters[0];
         iteration_status |= step_status;
         bool step_failed = step_status != 0;
-        irrecoverable = irrecoverable || (!adaptive && step_failed);
+        if (mixed_direct) {
+            // a failed step is repeated with direct FP64 Newton corrections:
+            // a fixed step once, at the same size; an adaptive one at the
+            // sizes the controller picks, until a step succeeds
+            const bool repeat = step_failed && (adaptive || !P.use_direct);
+            irrecoverable = irrecoverable || (!adaptive && step_failed && !repeat);
+            P.use_direct = repeat;
+        } else {
+            irrecoverable = irrecoverable || (!adaptive && step_failed);
+        }
         for (int i = 0; i < n_err; ++i) error[i] = step_failed ? T(1e16) : error[i];
 
         bool accept;

@@ -159,6 +159,62 @@ struct StageProblem {
         stage_state_in<V>(Y);
         jvp_in<T, V>(P, Y, in.p, in.d, v, jv, V(t));
     }
+    // The engine holds the dense Jacobian when n^2 <= 64 (its mixed mode's
+    // direct FP64 pass needs it).
+    bool holds_jacobian() const { return n * n <= 64; }
+    // (I - a h J) x = b by LU with partial pivoting in T (first largest pivot
+    // in each column), J being the FP32 matrix of the Krylov pass; false for
+    // a zero pivot or a non-finite solution.
+    bool direct_solve(T* x) const {
+        float Jf[MAXN * MAXN];
+        if (fused_jac()) {
+            rounded_jac<float>(Jf);
+        } else {
+            float Y[MAXN];
+            InputsIn<T, float> in;
+            in.bind(params, cfg->n_params, drivers, cfg->n_drivers);
+            stage_state_in<float>(Y);
+            for (int j = 0; j < n; ++j) {
+                float e[MAXN], col[MAXN];
+                for (int k = 0; k < n; ++k) e[k] = float(k == j ? 1 : 0);
+                sys_jvp<float>(P->sys, Y, in.p, in.d, e, col, float(t));
+                for (int i = 0; i < n; ++i) Jf[i * n + j] = col[i];
+            }
+        }
+        T M[MAXN * MAXN];
+        const T ah = a * h;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) M[i * n + j] = (i == j ? T(1) : T(0)) - ah * T(Jf[i * n + j]);
+        int piv[MAXN];
+        bool ok = true;
+        for (int k = 0; k < n; ++k) {
+            int p = k;
+            T best = std::fabs(M[k * n + k]);
+            for (int i = k + 1; i < n; ++i)
+                if (std::fabs(M[i * n + k]) > best) { best = std::fabs(M[i * n + k]); p = i; }
+            piv[k] = p;
+            if (p != k)
+                for (int j = 0; j < n; ++j) std::swap(M[k * n + j], M[p * n + j]);
+            const T d = M[k * n + k];
+            ok = ok && d != T(0);
+            const T inv = d != T(0) ? T(1) / d : T(0);
+            for (int i = k + 1; i < n; ++i) {
+                const T l = M[i * n + k] * inv;
+                M[i * n + k] = l;
+                for (int j = k + 1; j < n; ++j) M[i * n + j] -= l * M[k * n + j];
+            }
+        }
+        for (int k = 0; k < n; ++k)
+            if (piv[k] != k) std::swap(x[k], x[piv[k]]);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) x[i] -= M[i * n + j] * x[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) x[i] -= M[i * n + j] * x[j];
+            x[i] = x[i] / M[i * n + i];
+        }
+        for (int i = 0; i < n; ++i) ok = ok && std::isfinite(x[i]);
+        return ok;
+    }
     template <typename V>
     void op_in(const V* v, V* out) const {
         V jv[MAXN];
@@ -344,11 +400,19 @@ static int32_t linear_solve(const OracleConfig* cfg, Prob& Pb, int W, T* rhs, T*
     return mr_solve<T>(cfg, Pb, W, rhs, x, x_zero, it);
 }
 
+// Mixed mode (not in the reference), on problems that hold their dense
+// Jacobian (n^2 <= 64): when the loop repeats a step after a failed one
+// (Problem::use_direct), every correction is solved directly in FP64 (LU with
+// partial pivoting of I - a h J, J being the FP32 matrix of the Krylov
+// solves) from a zero starting increment.
 template <typename T, typename Prob>
 static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_store, int32_t* counters) {
     const T kappa = T(0.01), first_bound = T(1.0e-5), decay = T(0.3), div = T(2.0);
     const T huge = SC<T>::huge;
     const T stag = T(100.0 * std::sqrt(double(std::numeric_limits<T>::epsilon())));
+    const bool direct = cfg->mixed && Pb.holds_jacobian() && Pb.P->use_direct;
+    if (direct)
+        for (int i = 0; i < W; ++i) Pb.u[i] = T(0);
     T prev_theta = theta_store > T(0) ? theta_store : T(1);
     T ndz_prev = T(0);
     bool conv = false, failed = false;
@@ -363,9 +427,13 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         int kit = 0;
         int32_t lin;
         bool representable = true, progress = false;
-        if (cfg->mixed) {
-            // FP32 Krylov solve of the FP64 Newton system; the solvers leave
-            // the running residual in rl
+        if (direct) {
+            for (int i = 0; i < W; ++i) delta[i] = res[i];
+            lin = Pb.direct_solve(delta) ? ST_SUCCESS : ST_LINEAR_MAX_ITERS;
+            kit = 1;
+        } else if (cfg->mixed) {
+            // FP32 Krylov solve of the FP64 Newton system; the solvers
+            // leave the running residual in rl
             float rl[MAXW], dl[MAXW];
             for (int i = 0; i < W; ++i) { rl[i] = float(res[i]); dl[i] = 0.0f; }
             LinView<Prob, float> lv{&Pb, cfg};
@@ -383,13 +451,13 @@ static int32_t newton_solve(const OracleConfig* cfg, Prob& Pb, int W, T& theta_s
         T n2 = Pb.correction_norm2(delta);
         T ndz = T(std::sqrt(n2));
         bool judged = lin == ST_SUCCESS;
-        // Mixed mode only (not in the reference): an FP32 solve that ends in
-        // a BiCGSTAB breakdown, or stops at the iteration cap having lowered
-        // the residual norm, still yields a correction; it is applied, but
+        // Mixed mode only: an FP32 solve that ends in a BiCGSTAB
+        // breakdown, or stops at the iteration cap having lowered the
+        // residual norm, still yields a correction; it is applied, but
         // neither ends the iteration nor enters the contraction estimate.
-        // Any other unconverged FP32 solve, or a right-hand side whose FP32
-        // norm overflows, fails the Newton solve.
-        bool partial = cfg->mixed && !judged && representable &&
+        // Any other unconverged solve, or a right-hand side whose FP32
+        // norm overflows, fails the pass.
+        bool partial = cfg->mixed && !direct && !judged && representable &&
                        ((lin == ST_LINEAR_MAX_ITERS && progress) || lin == ST_BICGSTAB_BREAKDOWN);
         bool usable = (judged && representable) || partial;
         bool hist = ndz_prev > T(0);
@@ -638,6 +706,10 @@ struct UnsupportedStepper {
 // --------------------------------------------------------------------------
 template <typename T>
 struct FIRKProblem {
+    // the mixed mode keeps FP64 Krylov solves for the coupled stages: no
+    // direct pass
+    bool holds_jacobian() const { return false; }
+    bool direct_solve(T*) const { return false; }
     const Problem<T>* P;
     const OracleConfig* cfg;
     int s, n, nd;

@@ -161,10 +161,11 @@ def test_mixed_mode_kvaerno5_robertson_hard_runs(oracle_module):
     mode's FP32 BiCGSTAB returns, at its iteration cap, corrections of 1e9 to
     1e28 tolerance units. Such a solve is not applied unless it lowered its
     own residual, and a right-hand side whose FP32 norm overflows (a solve
-    that would "converge" at once on an infinite tolerance) fails the Newton
-    solve; the step is rejected and retried smaller, and every run finishes
-    within the tolerance-level distance of FP64 (applying those corrections
-    lands the runs on spurious stage solutions and ends them at dt_min)."""
+    that would "converge" at once on an infinite tolerance) fails the FP32
+    pass; the stage is then solved with direct FP64 corrections, and every
+    run finishes within the tolerance-level distance of FP64 (applying those
+    corrections lands the runs on spurious stage solutions and ends them at
+    dt_min)."""
     from systems import robertson, robertson_batch
     inits, params = robertson_batch(512, seed=45)
     pick = [264, 382, 402, 464]
