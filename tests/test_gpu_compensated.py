@@ -146,3 +146,33 @@ def test_robertson_radau_compensated_long_horizon(oracle_module):
     e = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
     print("radau robertson compensated long horizon: max", e.max())
     assert e.max() <= 1e-6
+
+
+def test_hh_compensated_direct_repeat_of_failed_steps(oracle_module):
+    """Mixed mode with the Krylov solves capped at one iteration (HH, SDIRK3
+    at a fixed step, 20 ms): FP32 passes fail on ~1 % of the steps, each
+    failed step is repeated with direct FP64 Newton corrections (LU of
+    I - a h J), and every run finishes, within 1e-6 of FP64 with converged
+    Krylov solves and of the oracle's restatement of the same mode."""
+    n = 64
+    inits, params = hh_batch(n, seed=41)
+    inits = inits.astype(np.float32).astype(np.float64)
+    params = params.astype(np.float32).astype(np.float64)
+    kw = dict(algorithm="l_stable_dirk_3", dt=2.0 ** -6, atol=1e-6, rtol=1e-6,
+              linear_correction_type="bicgstab", krylov_max_iters=1, save_every=1.0,
+              output_types=["state", "iteration_counters"])
+    solver = ca.Solver(hodgkin_huxley(np.float32), precision_mode="compensated", **kw)
+    solver._configure_drivers(hh_driver())
+    res = solver.solve(inits, params, duration=20.0, nan_error_trajectories=False)
+    assert np.all(res.status_codes == 0), res.status_messages
+    repeats = int(np.asarray(res.iteration_counters)[:, 3, :].sum())
+    print("hh direct repeats", repeats)
+    assert repeats > 100
+    ref = fp64_oracle(oracle_module, hodgkin_huxley(np.float64), dict(kw, krylov_max_iters=100),
+                      "hodgkin_huxley", inits, params, 20.0, hh_driver())
+    mixed = oracle_module.OracleRun(solver.kernel, "hodgkin_huxley", 20.0).solve(inits, params, threads=8)
+    st = res.state.astype(np.float64)
+    e64 = np.abs(st - ref["state"]).max() / np.abs(ref["state"]).max()
+    em = np.abs(st - mixed["state"]).max() / np.abs(mixed["state"]).max()
+    print("hh direct repeats: vs fp64", e64, "vs mixed oracle", em)
+    assert e64 <= 1e-6 and em <= 1e-6

@@ -180,3 +180,35 @@ def test_mixed_mode_kvaerno5_robertson_hard_runs(oracle_module):
     assert np.all(rm["status_codes"] == 0), rm["status_codes"]
     e = np.linalg.norm(rm["state"][-1] - r64["state"][-1], axis=0) / np.linalg.norm(r64["state"][-1], axis=0)
     assert e.max() <= 1e-5, e.max()
+
+
+def _hh_direct_case(kmax):
+    from systems import hodgkin_huxley, hh_batch, hh_driver
+    inits, params = hh_batch(64, seed=41)
+    inits = inits.astype(np.float32).astype(np.float64)
+    params = params.astype(np.float32).astype(np.float64)
+    kw = dict(algorithm="l_stable_dirk_3", dt=2.0 ** -6, atol=1e-6, rtol=1e-6,
+              linear_correction_type="bicgstab", krylov_max_iters=kmax, save_every=1.0,
+              output_types=["state", "iteration_counters"])
+    s32 = ca.Solver(hodgkin_huxley(np.float32), precision_mode="compensated", **kw)
+    s32._configure_drivers(hh_driver())
+    s64 = ca.Solver(hodgkin_huxley(np.float64), **dict(kw, krylov_max_iters=100))
+    s64._configure_drivers(hh_driver())
+    return inits, params, s32, s64
+
+
+def test_mixed_mode_direct_repeat_of_failed_fixed_steps(oracle_module):
+    """HH, SDIRK3 at a fixed step, mixed mode with the Krylov solves capped
+    at one iteration: the FP32 passes fail often (872 steps of 81,920), each
+    failed step is repeated with direct FP64 Newton corrections (LU of
+    I - a h J), and every run finishes, within 1e-6 of FP64 with converged
+    Krylov solves over 20 ms."""
+    inits, params, s32, s64 = _hh_direct_case(1)
+    rm = oracle_module.OracleRun(s32.kernel, "hodgkin_huxley", 20.0).solve(inits, params, threads=4)
+    r64 = oracle_module.OracleRun(s64.kernel, "hodgkin_huxley", 20.0).solve(inits, params, threads=4)
+    assert np.all(rm["status_codes"] == 0)
+    repeats = int(rm["iteration_counters"][:, 3, :].sum())
+    assert repeats > 100, repeats
+    assert int(r64["iteration_counters"][:, 3, :].sum()) == 0
+    e = np.abs(rm["state"] - r64["state"]).max() / np.abs(r64["state"]).max()
+    assert e <= 1e-6, e
