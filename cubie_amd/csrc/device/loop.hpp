@@ -157,6 +157,7 @@ struct Integrator {
         for (int i = 0; i < n; ++i) state[i] = real(inits[i * N + run]);
 #pragma unroll
         for (int i = 0; i < Cfg::n_params; ++i) ctx.params[i] = real(params[i * N + run]);
+        ctx.use_direct = false;   // a persistent lane's previous system may have left it set
 #pragma unroll
         for (int i = 0; i < Cfg::n_drivers; ++i) { ctx.drivers[i] = real(0); ctx.proposed_drivers[i] = real(0); }
 #pragma unroll
