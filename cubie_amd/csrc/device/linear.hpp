@@ -388,7 +388,7 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
     using L = typename Prob::L;
     constexpr int W = Prob::W;
     constexpr bool mixed = Cfg::newton_partial && !same_type<L, real>::value;
-    constexpr bool two_pass = mixed && Prob::direct_capable;
+    constexpr bool can_direct = mixed && Prob::direct_capable;
     const real kappa = real(0.01);
     const real first_bound = real(1.0e-5);
     const real theta_decay = real(0.3);
@@ -396,7 +396,7 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
     constexpr real huge = SolverConsts<real>::huge;
     constexpr real stag_eps = SolverConsts<real>::stagnation_eps;
     bool direct = false;
-    if constexpr (two_pass) direct = P.use_direct;
+    if constexpr (can_direct) direct = P.use_direct;
     if (direct) {
 #pragma unroll
         for (int i = 0; i < W; ++i) P.u[i] = real(0);
@@ -427,7 +427,7 @@ __device__ __forceinline__ int newton_solve(Prob& P, real& theta_store, int* cou
             lin = linear_solve<Cfg, L>(P, residual, delta, true, kit);
         } else {
             bool done = false;
-            if constexpr (two_pass) {
+            if constexpr (can_direct) {
                 if (direct) {
 #pragma unroll
                     for (int i = 0; i < W; ++i) delta[i] = -residual[i];
